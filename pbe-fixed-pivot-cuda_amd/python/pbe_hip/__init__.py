@@ -1,0 +1,161 @@
+"""Python front end of the gfx950 fixed-pivot PBE library (libpbe_hip.so).
+
+Mirrors the reference's C++ interface (include/pbe_cuda/aggregation.cuh and
+include/pbe_cuda/breakage.cuh): the same enum values, parameter structs with
+the same fields and defaults, and ``launch_aggregation_rhs`` /
+``launch_breakage_rhs`` that accumulate into a caller-zeroed ``rhs`` and return
+the HIP status code.  Calls go through the C ABI declared in
+include/pbe_hip/pbe_hip.h; there is no other implementation behind them --
+if the shared library is missing, importing this package raises.
+"""
+import ctypes
+import enum
+import os
+
+from . import hip
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+LIB_PATH = os.path.join(_PKG_ROOT, "lib", "libpbe_hip.so")
+
+
+class AggregationKernel(enum.IntEnum):
+    """include/pbe_cuda/aggregation.cuh:32-39"""
+    Constant = 0
+    Sum = 1
+    Product = 2
+    Brownian = 3
+    Shear = 4
+    BrownianShear = 5
+
+
+class BreakageSelection(enum.IntEnum):
+    """include/pbe_cuda/breakage.cuh:30-35"""
+    Constant = 0
+    Linear = 1
+    PowerLaw = 2
+    Threshold = 3
+
+
+class AggregationParams(ctypes.Structure):
+    """Layout of AggregationParams (include/pbe_cuda/aggregation.cuh:59-73)."""
+    _fields_ = [("n", ctypes.c_int), ("log_x0", ctypes.c_double),
+                ("inv_log_r", ctypes.c_double), ("kernel_type", ctypes.c_int),
+                ("beta0", ctypes.c_double), ("beta_br", ctypes.c_double),
+                ("beta_sh", ctypes.c_double), ("block_size", ctypes.c_int)]
+
+    def __init__(self, n=0, log_x0=0.0, inv_log_r=0.0, kernel_type=0, beta0=1.0,
+                 beta_br=0.0, beta_sh=0.0, block_size=256, kernel=None):
+        if kernel is not None:
+            kernel_type = kernel
+        super().__init__(int(n), float(log_x0), float(inv_log_r), int(kernel_type),
+                         float(beta0), float(beta_br), float(beta_sh), int(block_size))
+
+
+class BreakageParams(ctypes.Structure):
+    """Layout of BreakageParams (include/pbe_cuda/breakage.cuh:63-77)."""
+    _fields_ = [("n", ctypes.c_int), ("n_quad", ctypes.c_int),
+                ("selection", ctypes.c_int), ("S0", ctypes.c_double),
+                ("v_ref", ctypes.c_double), ("alpha", ctypes.c_double),
+                ("v_min", ctypes.c_double), ("block_size", ctypes.c_int)]
+
+    def __init__(self, n=0, n_quad=0, selection=0, S0=1.0, v_ref=1.0, alpha=1.0,
+                 v_min=0.0, block_size=256):
+        super().__init__(int(n), int(n_quad), int(selection), float(S0), float(v_ref),
+                         float(alpha), float(v_min), int(block_size))
+
+
+_lib = None
+
+
+def library():
+    """Load libpbe_hip.so (after the HIP runtime, so it binds to that one)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; "
+                "g.build()'` from the repository root")
+        hip.runtime()
+        L = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+        vp = ctypes.c_void_p
+        L.pbe_launch_aggregation_rhs.restype = ctypes.c_int
+        L.pbe_launch_aggregation_rhs.argtypes = [vp, vp, vp, ctypes.POINTER(AggregationParams), vp]
+        L.pbe_launch_breakage_rhs.restype = ctypes.c_int
+        L.pbe_launch_breakage_rhs.argtypes = [vp, vp, vp, vp, vp,
+                                              ctypes.POINTER(BreakageParams), vp]
+        L.pbe_version.restype = ctypes.c_char_p
+        L.pbe_version.argtypes = []
+        from . import _abi_ext
+        _abi_ext.declare(L)
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, hip.DeviceArray):
+        return a.ptr
+    if isinstance(a, ctypes.c_void_p):
+        return a
+    return ctypes.c_void_p(int(a))
+
+
+def _stream(s):
+    if s is None:
+        return None
+    return s.handle if isinstance(s, hip.Stream) else ctypes.c_void_p(int(s))
+
+
+def launch_aggregation_rhs(N, x, rhs, params, stream=None):
+    """rhs += aggregation source (pbe_launch_aggregation_rhs).  Returns the
+    HIP status code, as the reference's C++ function does."""
+    return library().pbe_launch_aggregation_rhs(_ptr(N), _ptr(x), _ptr(rhs),
+                                                ctypes.byref(params), _stream(stream))
+
+
+def launch_breakage_rhs(N, x, t_q, bw_q, rhs, params, stream=None):
+    """rhs += breakage source (pbe_launch_breakage_rhs).  Returns the HIP
+    status code."""
+    return library().pbe_launch_breakage_rhs(_ptr(N), _ptr(x), _ptr(t_q), _ptr(bw_q),
+                                             _ptr(rhs), ctypes.byref(params),
+                                             _stream(stream))
+
+
+def version():
+    return library().pbe_version().decode()
+
+
+def agg_params(n, agg):
+    """AggregationParams from a problems.* parameter dict."""
+    return AggregationParams(n=n, log_x0=agg.get("log_x0", 0.0),
+                             inv_log_r=agg.get("inv_log_r", 0.0),
+                             kernel_type=int(agg["kernel"]), beta0=agg.get("beta0", 1.0),
+                             beta_br=agg.get("beta_br", 0.0), beta_sh=agg.get("beta_sh", 0.0))
+
+
+def brk_params(n, n_quad, brk):
+    """BreakageParams from a problems.* parameter dict."""
+    return BreakageParams(n=n, n_quad=n_quad, selection=int(brk["selection"]),
+                          S0=brk.get("S0", 1.0), v_ref=brk.get("v_ref", 1.0),
+                          alpha=brk.get("alpha", 1.0), v_min=brk.get("v_min", 0.0))
+
+
+def rhs_on_device(N, x, agg=None, brk=None, t_q=None, bw_q=None):
+    """Convenience: upload host arrays, evaluate aggregation and/or breakage
+    into a zeroed rhs on the GPU, synchronise and return rhs on the host."""
+    import numpy as np
+    n = len(N)
+    dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
+    drhs = hip.DeviceArray(n)
+    drhs.zero()
+    if agg is not None:
+        hip.check(launch_aggregation_rhs(dN, dx, drhs, agg_params(n, agg)),
+                  "launch_aggregation_rhs")
+    if brk is not None:
+        dt, db = hip.DeviceArray.from_host(t_q), hip.DeviceArray.from_host(bw_q)
+        hip.check(launch_breakage_rhs(dN, dx, dt, db, drhs,
+                                      brk_params(n, len(np.atleast_1d(t_q)), brk)),
+                  "launch_breakage_rhs")
+    hip.synchronize()
+    return drhs.download()

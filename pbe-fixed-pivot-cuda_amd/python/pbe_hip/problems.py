@@ -1,0 +1,188 @@
+"""Host-side builders for pivot grids, initial conditions and daughter tables.
+
+The reference keeps these in its example harness, not in the library:
+grids ``examples/common/pbe_examples_utils.hpp:83-99``, initial conditions
+``:104-158``, daughter quadrature conventions ``include/pbe_cuda/breakage.cuh:43-50``
+and ``docs/api.md`` (symmetric ``{0.5}/{2}``, erosion ``{1-eps, eps}/{1, 1}``,
+uniform ``GL nodes / 2*GL weights``, power law ``2 x GL order``).
+
+Everything here is plain numpy on the host and produces float64 arrays that
+are handed unchanged to both the HIP library and the CPU oracle.
+"""
+import math
+
+import numpy as np
+
+# ---------------------------------------------------------------- grids ----
+
+
+def geometric_grid(n, x0, r):
+    """x[0] = x0, x[i] = x[i-1]*r by repeated multiplication, exactly as
+    ``make_geometric_grid`` (pbe_examples_utils.hpp:83-90)."""
+    x = np.empty(n)
+    x[0] = x0
+    for i in range(1, n):
+        x[i] = x[i - 1] * r
+    return x
+
+
+def geometric_grid_range(n, v_min, v_max):
+    """``make_geometric_grid_range`` (pbe_examples_utils.hpp:93-99)."""
+    r = (v_max / v_min) ** (1.0 / (n - 1))
+    return geometric_grid(n, v_min, r)
+
+
+def grid_lookup(x):
+    """(log_x0, inv_log_r) as the reference's callers compute them
+    (include/pbe_cuda/aggregation.cuh:105-108)."""
+    return math.log(x[0]), 1.0 / math.log(x[1] / x[0])
+
+
+def jittered_grid(n, b0, r, jitter, seed):
+    """Non-geometric grid: section boundaries b_i = b0*r^(i+d_i) with
+    d_i ~ U[-jitter, jitter] (jitter < 0.5 keeps them increasing) and pivots
+    x_i = sqrt(b_i b_{i+1}).  Callers pass inv_log_r = 0 for such a grid,
+    which selects the binary-search bracket path."""
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(-jitter, jitter, n + 1)
+    b = b0 * np.power(r, np.arange(n + 1) + d)
+    x = np.sqrt(b[:-1] * b[1:])
+    assert np.all(np.diff(x) > 0)
+    return x
+
+# ------------------------------------------------------ initial conditions --
+
+
+def exponential_ic_cells(x, N0, v0):
+    """N_i = N0*exp(-x_i/v0)*dx_i/v0 with geometric-midpoint cell width
+    dx_i = x_i(sqrt(r) - 1/sqrt(r)) (the cell convention of
+    pbe_examples_utils.hpp:125-133)."""
+    r = x[1] / x[0]
+    sr = math.sqrt(r)
+    dx = x * (sr - 1.0 / sr)
+    return N0 * np.exp(-x / v0) * dx / v0
+
+
+def exponential_ic_exact(x, N0, vc):
+    """``make_exponential_ic`` (pbe_examples_utils.hpp:120-136): exact cell
+    integral of (N0/vc) exp(-v/vc)."""
+    r = x[1] / x[0]
+    sr = math.sqrt(r)
+    return N0 * (np.exp(-(x / sr) / vc) - np.exp(-(x * sr) / vc))
+
+
+def lognormal_ic(x, x_med, sigma_ln, total, noise=0.0, seed=None):
+    """Lognormal number density in volume, following the pattern of
+    ``make_lognormal_ic_nd`` (pbe_examples_utils.hpp:142-158) with an explicit
+    median: N_i ~ exp(-0.5((ln x_i - ln x_med)/sigma)^2) x_i ln r, optional
+    seeded multiplicative noise U[1-noise, 1+noise], normalised to sum = total.
+    Bins whose density underflows are exactly zero."""
+    log_r = math.log(x[1] / x[0])
+    z = (np.log(x) - math.log(x_med)) / sigma_ln
+    N = np.exp(-0.5 * z * z) * x * log_r
+    if noise:
+        rng = np.random.default_rng(seed)
+        N = N * rng.uniform(1.0 - noise, 1.0 + noise, len(x))
+    return N * (total / N.sum())
+
+# ------------------------------------------------------- daughter tables ---
+
+
+def gauss_legendre(m, a=0.0, b=1.0):
+    """m-point Gauss-Legendre nodes and weights on (a, b)."""
+    t, w = np.polynomial.legendre.leggauss(m)
+    return 0.5 * (b - a) * t + 0.5 * (b + a), 0.5 * (b - a) * w
+
+
+def daughter_symmetric():
+    """b(v|v') = 2 delta(v - v'/2): t = {0.5}, bw = {2} (breakage.cuh:47)."""
+    return np.array([0.5]), np.array([2.0])
+
+
+def daughter_erosion(eps):
+    """b(v|v') = delta(v - eps v') + delta(v - (1-eps) v'):
+    t = {1-eps, eps}, bw = {1, 1} (breakage.cuh:48; flocculation example)."""
+    return np.array([1.0 - eps, eps]), np.array([1.0, 1.0])
+
+
+def daughter_uniform(n_quad):
+    """b(v|v') = 2/v': t = GL nodes on (0,1), bw = 2*GL weights
+    (examples/breakage_linear_uniform/main.cpp:76-80)."""
+    t, w = gauss_legendre(n_quad)
+    return t, 2.0 * w
+
+
+def daughter_power_law(q, n_quad):
+    """b(v|v') = C(v') v^q normalised by volume conservation,
+    C(v') = (q+2)/v'^(q+1), i.e. b = (q+2)/v' (v/v')^q, so that
+    bw = (q+2) t^q w.  Integrated with two Gauss-Legendre panels (0, 1/2) and
+    (1/2, 1) of n_quad/2 points each (breakage.cuh:50: n_quad = 2 x GL order).
+    The reference documents this table but ships no code for it."""
+    if n_quad % 2:
+        raise ValueError("power-law daughter needs an even n_quad (2 panels)")
+    m = n_quad // 2
+    t1, w1 = gauss_legendre(m, 0.0, 0.5)
+    t2, w2 = gauss_legendre(m, 0.5, 1.0)
+    t = np.concatenate([t1, t2])
+    w = np.concatenate([w1, w2])
+    return t, (q + 2.0) * np.power(t, q) * w
+
+
+def daughter_table(kind, n_quad=None, q=2.0, eps=0.05):
+    kind = kind.lower()
+    if kind in ("symmetric", "binary", "symmetric_binary"):
+        return daughter_symmetric()
+    if kind == "erosion":
+        return daughter_erosion(eps)
+    if kind == "uniform":
+        return daughter_uniform(n_quad)
+    if kind in ("powerlaw", "power_law"):
+        return daughter_power_law(q, n_quad)
+    raise ValueError(f"unknown daughter distribution {kind!r}")
+
+# ------------------------------------------------------------ workloads ---
+
+
+def config_smoluchowski(n=256):
+    """Constant-kernel aggregation on x_i = 1e-18 2^(i/4), beta0 = 1e-12,
+    N_i = N0 exp(-x_i/v0) dx_i/v0 with N0 = 1e12, v0 = 10 x_0."""
+    r = 2.0 ** 0.25
+    x = geometric_grid(n, 1e-18, r)
+    N = exponential_ic_cells(x, 1e12, 10.0 * x[0])
+    log_x0, inv_log_r = grid_lookup(x)
+    agg = dict(kernel=0, beta0=1e-12, beta_br=0.0, beta_sh=0.0,
+               log_x0=log_x0, inv_log_r=inv_log_r)
+    return x, N, agg
+
+
+def config_flocculation(n, seed=20261203, daughter="symmetric", n_quad=16,
+                        G=100.0):
+    """Single-instance aggregation + breakage: geometric grid r = 1.02 from
+    x_0 = 1e-18, Brownian+shear kernel beta_br = 6.73e-18, beta_sh = 4/3 G,
+    threshold breakage S0 = 1e-3 with v_min = x_{n/2}, symmetric-binary
+    daughters (or a GL uniform variant), lognormal N (median x_{n/4},
+    sigma_ln = 0.8, total 1e12, +-5 % seeded noise)."""
+    x = geometric_grid(n, 1e-18, 1.02)
+    N = lognormal_ic(x, x[n // 4], 0.8, 1e12, noise=0.05, seed=seed)
+    log_x0, inv_log_r = grid_lookup(x)
+    agg = dict(kernel=5, beta0=0.0, beta_br=6.73e-18, beta_sh=4.0 / 3.0 * G,
+               log_x0=log_x0, inv_log_r=inv_log_r)
+    brk = dict(selection=3, S0=1e-3, v_ref=1.0, alpha=1.0, v_min=x[n // 2])
+    if daughter == "symmetric":
+        t_q, bw_q = daughter_symmetric()
+    else:
+        t_q, bw_q = daughter_table(daughter, n_quad)
+    return x, N, agg, brk, t_q, bw_q
+
+
+def pair_units(n, n_quad):
+    """Work units of one aggregation + breakage evaluation: n(n+1)/2 pairs
+    plus n*n_quad breakage (parent, quadrature point) updates."""
+    return n * (n + 1) // 2 + n * n_quad
+
+
+# Algorithmic FP64 operations per pair for each aggregation kernel (FMA = 2),
+# and per breakage (parent, quad) update / per parent.
+PAIR_FLOPS = {0: 11, 1: 12, 2: 13, 3: 16, 4: 15, 5: 21}
+BREAKAGE_UPDATE_FLOPS = 8
+BREAKAGE_PARENT_FLOPS = 2

@@ -1,0 +1,187 @@
+"""GPU parity of the C-ABI entry points against the CPU oracle.
+
+Every case runs the HIP library through pbe_launch_aggregation_rhs /
+pbe_launch_breakage_rhs on seeded inputs and compares with the oracle on the
+same arrays: per entry |gpu - oracle| <= 1e-12 * A_i (A_i = sum of absolute
+contributions to bin i; bins with A_i = 0 must be exactly 0) and the first
+moment sum_i x_i rhs_i within 1e-13 * sum_i x_i A_i of the oracle's exact
+clip/daughter defect.
+"""
+import numpy as np
+import pytest
+
+from parity import assert_m1, assert_parity
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu(pbe, N, x, agg=None, brk=None, t_q=None, bw_q=None):
+    return pbe.rhs_on_device(N, x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+
+
+def _oracle(orc, N, x, agg=None, brk=None, t_q=None, bw_q=None):
+    ref = np.zeros(len(N))
+    A = np.zeros(len(N))
+    D = 0.0
+    if agg is not None:
+        r, a, d = orc.aggregation_rhs(N, x, **agg)
+        ref += r
+        A += a
+        D += d
+    if brk is not None:
+        r, a, d = orc.breakage_rhs(N, x, t_q, bw_q, **brk)
+        ref += r
+        A += a
+        D += d
+    return ref, A, D
+
+
+def test_smoluchowski_single_rhs(pbe, oracle_mod):
+    from pbe_hip import problems as P
+    x, N, agg = P.config_smoluchowski(256)
+    got = _gpu(pbe, N, x, agg=agg)
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg)
+    print("n=256 constant", assert_parity(got, ref, A, what="smoluchowski"))
+    assert_m1(got, x, A, D, what="smoluchowski")
+    # discrete identity of the constant kernel: sum rhs = -beta0 M0^2 / 2
+    M0 = N.sum()
+    assert abs(got.sum() + 0.5e-12 * M0 * M0) <= 1e-13 * (0.5e-12 * M0 * M0 * 3)
+
+
+@pytest.mark.parametrize("n", [1024, 4096])
+@pytest.mark.parametrize("daughter", ["symmetric", "uniform"])
+def test_flocculation_agg_brk(pbe, oracle_mod, n, daughter):
+    from pbe_hip import problems as P
+    x, N, agg, brk, t_q, bw_q = P.config_flocculation(n, daughter=daughter, n_quad=16)
+    got = _gpu(pbe, N, x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    w = assert_parity(got, ref, A, what=f"n={n} {daughter}")
+    m = assert_m1(got, x, A, D, what=f"n={n} {daughter}")
+    print(f"n={n} {daughter}: entry {w:.2e} m1 {m:.2e}")
+
+
+@pytest.mark.parametrize("kernel", range(6))
+def test_all_kernels_geometric(pbe, oracle_mod, kernel):
+    from pbe_hip import problems as P
+    n = 1500
+    x = P.geometric_grid(n, 1e-18, 1.02)
+    N = P.lognormal_ic(x, x[n // 3], 1.5, 1e12, noise=0.05, seed=7 + kernel)
+    log_x0, inv_log_r = P.grid_lookup(x)
+    agg = dict(kernel=kernel, beta0=1e-17 if kernel < 3 else 0.0,
+               beta_br=6.73e-18, beta_sh=1.0e-3, log_x0=log_x0, inv_log_r=inv_log_r)
+    if kernel == 2:
+        agg["beta0"] = 1e-17 / x[n // 3]
+    got = _gpu(pbe, N, x, agg=agg)
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg)
+    assert_parity(got, ref, A, what=f"kernel {kernel}")
+    assert_m1(got, x, A, D, what=f"kernel {kernel}")
+
+
+@pytest.mark.parametrize("kernel", range(6))
+def test_all_kernels_jittered_binary_search(pbe, oracle_mod, kernel):
+    """Non-geometric grid, inv_log_r = 0: the binary-search bracket path."""
+    from pbe_hip import problems as P
+    n = 3840
+    x = P.jittered_grid(n, 1e-18, 1.01, 0.2, seed=3840)
+    N = P.lognormal_ic(x, x[n // 4], 2.0, 1e12, noise=0.05, seed=11)
+    agg = dict(kernel=kernel, beta0=1e-17, beta_br=6.73e-18, beta_sh=1.0e-3,
+               log_x0=0.0, inv_log_r=0.0)
+    if kernel == 2:
+        agg["beta0"] = 1e-17 / x[n // 4]
+    got = _gpu(pbe, N, x, agg=agg)
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg)
+    assert_parity(got, ref, A, what=f"jittered kernel {kernel}")
+    assert_m1(got, x, A, D, what=f"jittered kernel {kernel}")
+
+
+DAUGHTERS = [("symmetric", None), ("erosion", None), ("uniform", 8), ("uniform", 16),
+             ("uniform", 32), ("powerlaw", 8), ("powerlaw", 16), ("powerlaw", 32)]
+
+
+@pytest.mark.parametrize("selection", range(4))
+@pytest.mark.parametrize("daughter,nq", DAUGHTERS)
+def test_breakage_jittered(pbe, oracle_mod, selection, daughter, nq):
+    from pbe_hip import problems as P
+    n = 3840
+    x = P.jittered_grid(n, 1e-18, 1.01, 0.2, seed=3840)
+    N = P.lognormal_ic(x, x[n // 2], 3.0, 1e12, noise=0.05, seed=5)
+    t_q, bw_q = P.daughter_table(daughter, nq, q=2.0, eps=0.05)
+    brk = dict(selection=selection, S0=1e-3, v_ref=x[n // 2], alpha=1.0 / 3.0,
+               v_min=x[n // 2])
+    got = _gpu(pbe, N, x, brk=brk, t_q=t_q, bw_q=bw_q)
+    ref, A, D = _oracle(oracle_mod, N, x, brk=brk, t_q=t_q, bw_q=bw_q)
+    assert_parity(got, ref, A, what=f"sel {selection} {daughter}{nq}")
+    assert_m1(got, x, A, D, what=f"sel {selection} {daughter}{nq}")
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 127, 129, 255, 257, 511, 513, 1000])
+def test_ragged_sizes(pbe, oracle_mod, n):
+    """Sizes around the tile edges, including n = 1 and 2 where every pair clips."""
+    from pbe_hip import problems as P
+    rng = np.random.default_rng(n)
+    x = P.geometric_grid(n, 1e-18, 1.05)
+    N = rng.uniform(0.0, 1e12, n)
+    log_x0 = np.log(x[0])
+    inv_log_r = 1.0 / np.log(1.05) if n > 1 else 0.0
+    agg = dict(kernel=5, beta0=0.0, beta_br=6.73e-18, beta_sh=1e-3, log_x0=log_x0,
+               inv_log_r=inv_log_r)
+    t_q, bw_q = P.daughter_uniform(8)
+    brk = dict(selection=2, S0=1e-3, v_ref=x[n // 2], alpha=0.5, v_min=0.0)
+    got = _gpu(pbe, N, x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    assert_parity(got, ref, A, what=f"n={n}")
+
+
+def test_negative_and_zero_populations(pbe, oracle_mod):
+    """Aggregation has no guard (negative N contributes); breakage skips
+    N <= 2.2e-308 (src/breakage_kernels.cuh:131,195)."""
+    from pbe_hip import problems as P
+    n = 700
+    rng = np.random.default_rng(1)
+    x = P.geometric_grid(n, 1e-18, 1.03)
+    N = rng.normal(0.0, 1e10, n)
+    N[rng.uniform(size=n) < 0.3] = 0.0
+    N[5] = 1e-310
+    log_x0, inv_log_r = P.grid_lookup(x)
+    agg = dict(kernel=4, beta0=0.0, beta_br=0.0, beta_sh=1e-3, log_x0=log_x0,
+               inv_log_r=inv_log_r)
+    t_q, bw_q = P.daughter_erosion(0.05)
+    brk = dict(selection=1, S0=1e-3, v_ref=x[n // 2], alpha=1.0, v_min=0.0)
+    got = _gpu(pbe, N, x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    assert_parity(got, ref, A, what="negative N")
+
+
+def test_accumulates_into_rhs(pbe, oracle_mod):
+    """rhs is accumulated into, not overwritten (aggregation.cuh:88-91)."""
+    from pbe_hip import hip, problems as P
+    x, N, agg, brk, t_q, bw_q = P.config_flocculation(1024)
+    n = len(N)
+    base = np.linspace(-1.0, 1.0, n) * 1e3
+    dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
+    drhs = hip.DeviceArray.from_host(base)
+    dt, db = hip.DeviceArray.from_host(t_q), hip.DeviceArray.from_host(bw_q)
+    hip.check(pbe.launch_breakage_rhs(dN, dx, dt, db, drhs,
+                                      pbe.brk_params(n, len(t_q), brk)))
+    hip.check(pbe.launch_aggregation_rhs(dN, dx, drhs, pbe.agg_params(n, agg)))
+    hip.synchronize()
+    got = drhs.download()
+    ref, A, _ = _oracle(oracle_mod, N, x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    assert_parity(got - base, ref, A + np.abs(base) * 1e-4, what="accumulate")
+
+
+def test_flocculation_16384_vs_port(pbe, oracle_mod):
+    """Full-size workload: every entry against the OpenMP port (plain FP64,
+    reference bracket rule) and the first-moment identity."""
+    from pbe_hip import problems as P
+    x, N, agg, brk, t_q, bw_q = P.config_flocculation(16384)
+    got = _gpu(pbe, N, x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    port = oracle_mod.port_aggregation_rhs(N, x, threads=16, **agg)
+    port += oracle_mod.port_breakage_rhs(N, x, t_q, bw_q, threads=16, **brk)
+    scale = np.abs(port).max()
+    assert np.all(np.abs(got - port) <= 1e-10 * scale + 1e-9 * np.abs(port))
+    # the exact oracle on the same input (double-double, ~10 s on one core)
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    w = assert_parity(got, ref, A, what="n=16384")
+    m = assert_m1(got, x, A, D, what="n=16384")
+    print(f"n=16384: entry {w:.2e} m1 {m:.2e}")
