@@ -103,6 +103,137 @@ hipError_t pbe_launch_breakage_rhs(const double* N, const double* x,
                                    const pbe_breakage_params* params,
                                    hipStream_t stream);
 
+/* ------------------------------------------------------------------------ */
+/* Device time integration (no counterpart in the reference library: it     */
+/* replaces the host-side euler_step / rk4_step of the reference's example  */
+/* harness, examples/common/pbe_examples_utils.hpp:391-457, with the same   */
+/* arithmetic and evaluation order, and no host round-trips).               */
+/* ------------------------------------------------------------------------ */
+enum { PBE_METHOD_RK4 = 0, PBE_METHOD_EULER = 1 };
+
+/* Bytes of device workspace an integrator needs for n bins (RK4: 3n doubles,
+ * Euler: n doubles); 0 for n <= 0. */
+size_t pbe_integrator_workspace_bytes(int n, int method);
+
+/*
+ * Advance N in place by `steps` steps of size dt of dN/dt = R_agg + R_brk,
+ * where agg or brk may be NULL (that source term is then absent).  x, t_q,
+ * bw_q as for the launch functions; workspace: caller-owned device memory of
+ * at least pbe_integrator_workspace_bytes(n, method) bytes.  Enqueue-only.
+ */
+hipError_t pbe_integrate(double* N, const double* x, const double* t_q,
+                         const double* bw_q, const pbe_aggregation_params* agg,
+                         const pbe_breakage_params* brk, double dt, int steps,
+                         int method, void* workspace, size_t workspace_bytes,
+                         hipStream_t stream);
+
+/* A hipGraph holding `steps_per_launch` integrator steps (all pointers and
+ * parameters are baked in at creation; the graph contains no memset node). */
+typedef struct pbe_graph_s* pbe_graph;
+hipError_t pbe_integrator_graph_create(pbe_graph* out, double* N, const double* x,
+                                       const double* t_q, const double* bw_q,
+                                       const pbe_aggregation_params* agg,
+                                       const pbe_breakage_params* brk, double dt,
+                                       int steps_per_launch, int method,
+                                       void* workspace, size_t workspace_bytes);
+hipError_t pbe_graph_launch(pbe_graph graph, hipStream_t stream);
+hipError_t pbe_graph_destroy(pbe_graph graph);
+
+/* ------------------------------------------------------------------------ */
+/* Batched form: many independent instances of uneven n in one persistent   */
+/* launch (no counterpart in the reference, which evaluates one instance    */
+/* per call).  All instances share the aggregation kernel type, the         */
+/* selection function and the daughter table (t_q, bw_q); each has its own  */
+/* grid, coefficients and slice [offset, offset + n) of the packed N, x and */
+/* rhs arrays.                                                              */
+/* ------------------------------------------------------------------------ */
+typedef struct pbe_batch_instance {
+    int    n;          /* bins of this instance */
+    int    offset;     /* index of its bin 0 in the packed arrays */
+    double log_x0;     /* as AggregationParams */
+    double inv_log_r;  /* 0: binary-search bracket */
+    double beta0, beta_br, beta_sh;   /* aggregation coefficients */
+    double S0, v_ref, alpha, v_min;   /* breakage selection parameters */
+} pbe_batch_instance;
+
+typedef struct pbe_batch_plan_s* pbe_batch_plan;
+
+/* Build the work list (pair tiles and parent blocks of every instance,
+ * longest first) and upload it with the descriptors.  kernel_type = -1 means
+ * no aggregation, selection = -1 no breakage.  The plan owns device memory
+ * until pbe_batch_plan_destroy. */
+hipError_t pbe_batch_plan_create(pbe_batch_plan* out,
+                                 const pbe_batch_instance* instances, int count,
+                                 int kernel_type, int selection, int n_quad);
+hipError_t pbe_batch_plan_destroy(pbe_batch_plan plan);
+/* Length of the packed arrays the plan addresses (max offset + n). */
+int pbe_batch_plan_size(pbe_batch_plan plan);
+
+/* rhs += R_agg + R_brk of every instance (packed arrays; rhs zeroed by the
+ * caller).  Enqueue-only. */
+hipError_t pbe_launch_batched_rhs(pbe_batch_plan plan, const double* N,
+                                  const double* x, const double* t_q,
+                                  const double* bw_q, double* rhs,
+                                  hipStream_t stream);
+
+/* Integrate every instance of the plan (workspace sized for
+ * pbe_batch_plan_size(plan) bins), eagerly or as a hipGraph. */
+hipError_t pbe_batch_integrate(pbe_batch_plan plan, double* N, const double* x,
+                               const double* t_q, const double* bw_q, double dt,
+                               int steps, int method, void* workspace,
+                               size_t workspace_bytes, hipStream_t stream);
+hipError_t pbe_batch_integrator_graph_create(pbe_graph* out, pbe_batch_plan plan,
+                                             double* N, const double* x,
+                                             const double* t_q, const double* bw_q,
+                                             double dt, int steps_per_launch,
+                                             int method, void* workspace,
+                                             size_t workspace_bytes);
+
+/* ------------------------------------------------------------------------ */
+/* Differentiation (no counterpart in the reference): hand-written          */
+/* vector-Jacobian products of the right-hand side and the discrete adjoint */
+/* of explicit RK4 with checkpoint-and-recompute.  Gradient slots of        */
+/* `gtheta` (5 doubles, device memory): d/d beta0, beta_br, beta_sh, S0,    */
+/* alpha.  All results are atomic-free and bitwise repeatable.              */
+/* ------------------------------------------------------------------------ */
+enum { PBE_GRAD_BETA0 = 0, PBE_GRAD_BETA_BR = 1, PBE_GRAD_BETA_SH = 2,
+       PBE_GRAD_S0 = 3, PBE_GRAD_ALPHA = 4, PBE_GRAD_COUNT = 5 };
+
+size_t pbe_vjp_workspace_bytes(int n);
+
+/* gN = lam^T dR/dN and gtheta = lam^T dR/dtheta at N (both overwritten),
+ * R = R_agg + R_brk as for pbe_integrate. */
+hipError_t pbe_rhs_vjp(const double* N, const double* x, const double* t_q,
+                       const double* bw_q, const double* lam,
+                       const pbe_aggregation_params* agg,
+                       const pbe_breakage_params* brk, double* gN, double* gtheta,
+                       void* workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* Workspace of the RK4 adjoint: ceil(steps/every) checkpoints plus the 4
+ * stage states of one segment of `every` steps, plus O(n) scratch. */
+size_t pbe_adjoint_workspace_bytes(int n, int steps, int checkpoint_every);
+
+/* Forward pass: advance N in place by `steps` RK4 steps, storing a
+ * checkpoint every `checkpoint_every` steps in the workspace. */
+hipError_t pbe_adjoint_rk4_forward(double* N, const double* x, const double* t_q,
+                                   const double* bw_q,
+                                   const pbe_aggregation_params* agg,
+                                   const pbe_breakage_params* brk, double dt,
+                                   int steps, int checkpoint_every, void* workspace,
+                                   size_t workspace_bytes, hipStream_t stream);
+
+/* Reverse pass after pbe_adjoint_rk4_forward with the same problem and
+ * workspace: given gN_final = dL/dN(t_end), writes gN0 = dL/dN(0) and
+ * gtheta = dL/dtheta, recomputing each segment from its checkpoint. */
+hipError_t pbe_adjoint_rk4_backward(const double* gN_final, double* gN0,
+                                    double* gtheta, const double* x,
+                                    const double* t_q, const double* bw_q,
+                                    const pbe_aggregation_params* agg,
+                                    const pbe_breakage_params* brk, double dt,
+                                    int steps, int checkpoint_every,
+                                    void* workspace, size_t workspace_bytes,
+                                    hipStream_t stream);
+
 /* Library version string, e.g. "pbe_hip 0.1.0 gfx950". */
 const char* pbe_version(void);
 

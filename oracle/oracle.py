@@ -194,3 +194,81 @@ def port_breakage_rhs(N, x, t_q, bw_q, selection, S0=1.0, v_ref=1.0,
     if e:
         raise OracleError(e)
     return rhs
+
+
+# ------------------------------------------------------------ complex step --
+class _CsProblem(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("x", _dp), ("use_agg", ctypes.c_int),
+                ("kernel", ctypes.c_int), ("use_brk", ctypes.c_int), ("nq", ctypes.c_int),
+                ("t_q", _dp), ("bw_q", _dp), ("sel", ctypes.c_int),
+                ("v_ref", ctypes.c_double), ("v_min", ctypes.c_double),
+                ("threads", ctypes.c_int)]
+
+
+def _cs_setup(x, agg, brk, t_q, bw_q, threads):
+    x = _f64(x)
+    keep = [x]
+    p = _CsProblem()
+    p.n = len(x)
+    p.x = _p(x)
+    p.threads = int(threads)
+    th = np.zeros(5)
+    if agg:
+        p.use_agg, p.kernel = 1, int(agg["kernel"])
+        th[0], th[1], th[2] = agg.get("beta0", 0.0), agg.get("beta_br", 0.0), agg.get("beta_sh", 0.0)
+    if brk:
+        tq, bq = _f64(t_q), _f64(bw_q)
+        keep += [tq, bq]
+        p.use_brk, p.nq, p.t_q, p.bw_q = 1, len(tq), _p(tq), _p(bq)
+        p.sel = int(brk["selection"])
+        p.v_ref, p.v_min = brk.get("v_ref", 1.0), brk.get("v_min", 0.0)
+        th[3], th[4] = brk.get("S0", 1.0), brk.get("alpha", 1.0)
+    return p, th, keep
+
+
+def _cs_fns():
+    L = lib()
+    if not hasattr(L, "_cs_ready"):
+        L.cs_rhs.restype = ctypes.c_int
+        L.cs_rhs.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.cs_rk4_loss.restype = ctypes.c_int
+        L.cs_rk4_loss.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                  ctypes.c_int, _dp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp]
+        L._cs_ready = True
+    return L
+
+
+THETA_NAMES = ("beta0", "beta_br", "beta_sh", "S0", "alpha")
+
+
+def cs_rhs(N, x, agg=None, brk=None, t_q=None, bw_q=None, dN=None, dtheta=None,
+           h=1e-30, threads=1):
+    """Complex-step directional derivative of R at (N, theta) along
+    (dN, dtheta): returns (R, dR) with dR = Im R(N + i h dN, theta + i h dtheta)/h."""
+    p, th, keep = _cs_setup(x, agg, brk, t_q, bw_q, threads)
+    n = len(N)
+    Nim = np.zeros(n) if dN is None else h * _f64(dN)
+    thim = np.zeros(5) if dtheta is None else h * _f64(dtheta)
+    ore, oim = np.zeros(n), np.zeros(n)
+    e = _cs_fns().cs_rhs(ctypes.byref(p), _p(th), _p(thim), _p(_f64(N)), _p(Nim),
+                         _p(ore), _p(oim))
+    if e:
+        raise OracleError(e)
+    return ore, oim / h
+
+
+def cs_loss(N0, x, dt, steps, logT, mask, eps, agg=None, brk=None, t_q=None, bw_q=None,
+            dN0=None, dtheta=None, h=1e-30, threads=1):
+    """Complex-step derivative of the RK4 log-misfit loss along (dN0, dtheta):
+    returns (L, dL)."""
+    p, th, keep = _cs_setup(x, agg, brk, t_q, bw_q, threads)
+    n = len(N0)
+    Nim = np.zeros(n) if dN0 is None else h * _f64(dN0)
+    thim = np.zeros(5) if dtheta is None else h * _f64(dtheta)
+    Lr, Li = ctypes.c_double(), ctypes.c_double()
+    e = _cs_fns().cs_rk4_loss(ctypes.byref(p), _p(th), _p(thim), _p(_f64(N0)), _p(Nim),
+                              dt, int(steps), _p(_f64(logT)), _p(_f64(mask)), eps,
+                              ctypes.byref(Lr), ctypes.byref(Li), None, None)
+    if e:
+        raise OracleError(e)
+    return Lr.value, Li.value / h

@@ -1,0 +1,278 @@
+// batch.hip -- thousands of independent PBE instances of uneven n in one
+// persistent launch (no counterpart in the reference, which evaluates one
+// instance per call).
+//
+// A plan, built once on the host, lists every unit of work of every instance
+// -- aggregation pair tiles (T = 128) and breakage parent blocks (256
+// parents) -- sorts them by estimated cost (longest first) and uploads them
+// with per-instance descriptors (size, offset into the packed arrays, own
+// grid lookup and own coefficients).  The persistent kernels run a fixed grid
+// sized to the device; workgroup b takes items b, b + G, b + 2G, ... of the
+// sorted list, which balances the load without a device-side queue (and keeps
+// the launch free of counters that would need resetting under graph replay).
+// Each item runs exactly the single-instance tile / block code.
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "agg_tile.hpp"
+#include "brk_block.hpp"
+#include "pbe_internal.hpp"
+
+namespace pbe {
+
+constexpr int kBatchU = 2;                      // 128 x 128 pair tiles
+constexpr int kBatchT = AggShape<kBatchU>::T;
+
+struct BatchInstDev {
+    int n, off, nb, pad;
+    double log_x0, inv_log_r, b0, b_br, b_sh, S0, v_ref, alpha, v_min;
+};
+
+struct BatchItem {
+    int inst;
+    int unit;       // tile index (diagonal-major) or parent block index
+};
+
+struct BatchArgs {
+    const double* __restrict__ N;
+    const double* __restrict__ x;
+    const double* __restrict__ t_q;
+    const double* __restrict__ bw_q;
+    double* rhs;
+    const BatchInstDev* __restrict__ inst;
+    const BatchItem* __restrict__ items;
+    int nitems;
+    int nq;
+};
+
+template <int KT>
+__global__ __launch_bounds__(256)
+void agg_batch_kernel(BatchArgs b)
+{
+    __shared__ AggSmem<KT, kBatchU> sm;
+    for (int it = blockIdx.x; it < b.nitems; it += gridDim.x) {
+        const BatchItem w = b.items[it];
+        const BatchInstDev d = b.inst[w.inst];
+        AggArgs a;
+        a.N = b.N + d.off;
+        a.x = b.x + d.off;
+        a.rhs = b.rhs + d.off;
+        a.n = d.n;
+        a.nb = d.nb;
+        a.coef = KernelCoef{d.b0, d.b_br, d.b_sh};
+        a.log_x0 = d.log_x0;
+        a.inv_log_r = d.inv_log_r;
+        int p, dg;
+        decode_tile(w.unit, d.nb, p, dg);
+        agg_tile_body<KT, kBatchU>(a, p * kBatchT, (p + dg) * kBatchT, dg == 0, sm);
+        __syncthreads();
+    }
+}
+
+template <int SEL>
+__global__ __launch_bounds__(256)
+void brk_batch_kernel(BatchArgs b)
+{
+    __shared__ BrkSmem sm;
+    for (int it = blockIdx.x; it < b.nitems; it += gridDim.x) {
+        const BatchItem w = b.items[it];
+        const BatchInstDev d = b.inst[w.inst];
+        BrkArgs a{b.N + d.off, b.x + d.off, b.t_q, b.bw_q, b.rhs + d.off, d.n, b.nq,
+                  d.S0, d.v_ref, d.alpha, d.v_min};
+        brk_block_body<SEL>(a, w.unit * kBrkParents, sm);
+        __syncthreads();
+    }
+}
+
+}  // namespace pbe
+
+struct pbe_batch_plan_s {
+    int count = 0;
+    int total = 0;              // max(offset + n) over instances
+    int kernel_type = -1;       // -1: no aggregation
+    int selection = -1;         // -1: no breakage
+    int n_quad = 0;
+    int n_agg_items = 0, n_brk_items = 0;
+    int grid = 0;
+    pbe::BatchInstDev* d_inst = nullptr;
+    pbe::BatchItem* d_agg_items = nullptr;
+    pbe::BatchItem* d_brk_items = nullptr;
+};
+
+namespace pbe {
+
+static double tile_cost(int n, int t)
+{
+    const long long nb = (n + kBatchT - 1) / kBatchT;
+    // decode as the device does
+    long long d = 0;
+    while (d + 1 < nb && (d + 1) * nb - (d + 1) * d / 2 <= t) ++d;
+    const long long p = t - (d * nb - d * (d - 1) / 2);
+    const long long j0 = p * kBatchT, k0 = (p + d) * kBatchT;
+    const double rows = (double)std::min<long long>(kBatchT, n - j0);
+    const double cols = (double)std::min<long long>(kBatchT, n - k0);
+    if (d == 0) return 2.0 * rows * (rows + 1) * 0.5 + 2.0 * n;  // mixed + death pass
+    if (d == 1) return 1.5 * rows * cols;                        // near band spill
+    return rows * cols;
+}
+
+hipError_t batch_plan_create(pbe_batch_plan* out, const pbe_batch_instance* hi, int count,
+                             int kernel_type, int selection, int n_quad)
+{
+    if (!out) return hipErrorInvalidValue;
+    *out = nullptr;
+    if (count <= 0 || !hi) return hipErrorInvalidValue;
+    if (kernel_type < -1 || kernel_type > 5 || selection < -1 || selection > 3)
+        return hipErrorInvalidValue;
+    if (kernel_type < 0 && selection < 0) return hipErrorInvalidValue;
+    if (selection >= 0 && n_quad <= 0) return hipErrorInvalidValue;
+
+    std::vector<BatchInstDev> inst(count);
+    long long total = 0;
+    for (int i = 0; i < count; ++i) {
+        const pbe_batch_instance& s = hi[i];
+        if (s.n <= 0 || s.offset < 0) return hipErrorInvalidValue;
+        total = std::max<long long>(total, (long long)s.offset + s.n);
+        if (total > 0x7fffffffLL) return hipErrorInvalidValue;
+        BatchInstDev& d = inst[i];
+        d.n = s.n;
+        d.off = s.offset;
+        d.nb = (s.n + kBatchT - 1) / kBatchT;
+        d.pad = 0;
+        d.log_x0 = s.log_x0;
+        d.inv_log_r = s.inv_log_r;
+        d.b0 = s.beta0;
+        d.b_br = s.beta_br;
+        d.b_sh = s.beta_sh;
+        d.S0 = s.S0;
+        d.v_ref = s.v_ref;
+        d.alpha = s.alpha;
+        d.v_min = s.v_min;
+    }
+
+    struct Costed { double c; BatchItem w; };
+    std::vector<Costed> agg, brk;
+    if (kernel_type >= 0) {
+        for (int i = 0; i < count; ++i) {
+            const long long nb = inst[i].nb;
+            const long long tiles = nb * (nb + 1) / 2;
+            for (long long t = 0; t < tiles; ++t)
+                agg.push_back({tile_cost(inst[i].n, (int)t), BatchItem{i, (int)t}});
+        }
+    }
+    if (selection >= 0) {
+        for (int i = 0; i < count; ++i) {
+            const int blocks = (inst[i].n + kBrkParents - 1) / kBrkParents;
+            for (int b = 0; b < blocks; ++b)
+                brk.push_back({(double)std::min(kBrkParents, inst[i].n - b * kBrkParents) * n_quad,
+                               BatchItem{i, b}});
+        }
+    }
+    auto by_cost = [](const Costed& a, const Costed& b) {
+        if (a.c != b.c) return a.c > b.c;
+        if (a.w.inst != b.w.inst) return a.w.inst < b.w.inst;
+        return a.w.unit < b.w.unit;
+    };
+    std::sort(agg.begin(), agg.end(), by_cost);
+    std::sort(brk.begin(), brk.end(), by_cost);
+
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            cus = v;
+    }
+
+    auto* p = new (std::nothrow) pbe_batch_plan_s();
+    if (!p) return hipErrorOutOfMemory;
+    p->count = count;
+    p->total = (int)total;
+    p->kernel_type = kernel_type;
+    p->selection = selection;
+    p->n_quad = n_quad;
+    p->n_agg_items = (int)agg.size();
+    p->n_brk_items = (int)brk.size();
+    p->grid = cus * 4;
+
+    std::vector<BatchItem> items;
+    hipError_t e = hipMalloc(&p->d_inst, sizeof(BatchInstDev) * count);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->d_inst, inst.data(), sizeof(BatchInstDev) * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !agg.empty()) {
+        items.clear();
+        for (auto& c : agg) items.push_back(c.w);
+        e = hipMalloc(&p->d_agg_items, sizeof(BatchItem) * items.size());
+        if (e == hipSuccess)
+            e = hipMemcpy(p->d_agg_items, items.data(), sizeof(BatchItem) * items.size(),
+                          hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && !brk.empty()) {
+        items.clear();
+        for (auto& c : brk) items.push_back(c.w);
+        e = hipMalloc(&p->d_brk_items, sizeof(BatchItem) * items.size());
+        if (e == hipSuccess)
+            e = hipMemcpy(p->d_brk_items, items.data(), sizeof(BatchItem) * items.size(),
+                          hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        (void)batch_plan_destroy(p);
+        return e;
+    }
+    *out = p;
+    return hipSuccess;
+}
+
+hipError_t batch_plan_destroy(pbe_batch_plan p)
+{
+    if (!p) return hipErrorInvalidValue;
+    hipError_t e = hipSuccess;
+    for (void* q : {(void*)p->d_inst, (void*)p->d_agg_items, (void*)p->d_brk_items}) {
+        if (q) {
+            hipError_t e2 = hipFree(q);
+            if (e == hipSuccess) e = e2;
+        }
+    }
+    delete p;
+    return e;
+}
+
+int batch_plan_total(pbe_batch_plan p) { return p ? p->total : 0; }
+
+bool batch_plan_needs_table(pbe_batch_plan p) { return p && p->selection >= 0; }
+
+hipError_t enqueue_batched_rhs(pbe_batch_plan p, const double* N, const double* x,
+                               const double* t_q, const double* bw_q, double* rhs,
+                               hipStream_t s)
+{
+    BatchArgs a{N, x, t_q, bw_q, rhs, p->d_inst, nullptr, 0, p->n_quad};
+    if (p->kernel_type >= 0 && p->n_agg_items > 0) {
+        a.items = p->d_agg_items;
+        a.nitems = p->n_agg_items;
+        const dim3 g((unsigned)std::min(p->grid, p->n_agg_items)), b(256);
+        switch (p->kernel_type) {
+        case 0: hipLaunchKernelGGL(agg_batch_kernel<0>, g, b, 0, s, a); break;
+        case 1: hipLaunchKernelGGL(agg_batch_kernel<1>, g, b, 0, s, a); break;
+        case 2: hipLaunchKernelGGL(agg_batch_kernel<2>, g, b, 0, s, a); break;
+        case 3: hipLaunchKernelGGL(agg_batch_kernel<3>, g, b, 0, s, a); break;
+        case 4: hipLaunchKernelGGL(agg_batch_kernel<4>, g, b, 0, s, a); break;
+        case 5: hipLaunchKernelGGL(agg_batch_kernel<5>, g, b, 0, s, a); break;
+        default: return hipErrorInvalidValue;
+        }
+    }
+    if (p->selection >= 0 && p->n_brk_items > 0) {
+        a.items = p->d_brk_items;
+        a.nitems = p->n_brk_items;
+        const dim3 g((unsigned)std::min(p->grid, p->n_brk_items)), b(256);
+        switch (p->selection) {
+        case 0: hipLaunchKernelGGL(brk_batch_kernel<0>, g, b, 0, s, a); break;
+        case 1: hipLaunchKernelGGL(brk_batch_kernel<1>, g, b, 0, s, a); break;
+        case 2: hipLaunchKernelGGL(brk_batch_kernel<2>, g, b, 0, s, a); break;
+        case 3: hipLaunchKernelGGL(brk_batch_kernel<3>, g, b, 0, s, a); break;
+        default: return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
+}
+
+}  // namespace pbe

@@ -20,4 +20,62 @@ hipError_t launch_breakage(const double* N, const double* x, const double* t_q,
 // Columns per lane picked for an aggregation problem of size n.
 int default_tile_u(int n, int kernel_type);
 
+// One right-hand side R(N) = R_agg(N) + R_brk(N); either part may be absent.
+// With `batch` set, R is the batched source of every instance of the plan and
+// n is the packed length.
+struct RhsProblem {
+    int n;
+    const double* x;
+    const double* t_q;
+    const double* bw_q;
+    const pbe_aggregation_params* agg;
+    const pbe_breakage_params* brk;
+    pbe_batch_plan batch;
+};
+
+hipError_t batch_plan_create(pbe_batch_plan* out, const pbe_batch_instance* instances,
+                             int count, int kernel_type, int selection, int n_quad);
+hipError_t batch_plan_destroy(pbe_batch_plan plan);
+int batch_plan_total(pbe_batch_plan plan);
+bool batch_plan_needs_table(pbe_batch_plan plan);
+hipError_t enqueue_batched_rhs(pbe_batch_plan plan, const double* N, const double* x,
+                               const double* t_q, const double* bw_q, double* rhs,
+                               hipStream_t s);
+
+hipError_t enqueue_fill_zero(double* a, int n, hipStream_t s);
+// rhs (zero on entry) += R(Nin)
+hipError_t enqueue_rhs(const RhsProblem& p, const double* Nin, double* rhs, hipStream_t s);
+// One classic RK4 step in place on N; work = 3n doubles whose first n (k) are
+// zero on entry and are left zero on exit.
+hipError_t enqueue_rk4_step(const RhsProblem& p, double* N, double dt, double* work,
+                            hipStream_t s);
+// As above, with the stage states Y2, Y3, Y4 written to stages[0..2] (NULL:
+// a scratch slot of work).
+hipError_t enqueue_rk4_step_stages(const RhsProblem& p, double* N, double dt, double* work,
+                                   double* const* stages, hipStream_t s);
+// One explicit Euler step; work = n doubles, zero on entry and exit.
+hipError_t enqueue_euler_step(const RhsProblem& p, double* N, double dt, double* work,
+                              hipStream_t s);
+
+// ----------------------------------------------------------- adjoint ----
+hipError_t enqueue_copy(const double* src, double* dst, int n, hipStream_t s);
+hipError_t enqueue_vjp_table(const double* x, double* tab, int n, hipStream_t s);
+// gN = lam . dR/dN at N (overwritten); gth_rows[n][5] += per-row parameter
+// partials (beta0, beta_br, beta_sh, S0, alpha); tab from enqueue_vjp_table.
+hipError_t enqueue_vjp(const RhsProblem& p, const double* N, const double* lam,
+                       const double* tab, double* gN, double* gth_rows, hipStream_t s);
+
+// Workspace of the checkpointed RK4 adjoint, offsets in doubles.
+struct AdjointLayout {
+    int nseg;
+    size_t ckpt, seg, tab, lam, lam2, av, mu, rk, gth, total;
+};
+AdjointLayout adjoint_layout(int n, int steps, int every);
+hipError_t enqueue_adjoint_forward(const RhsProblem& p, double* N, double dt, int steps,
+                                   int every, double* ws, hipStream_t s);
+hipError_t enqueue_adjoint_backward(const RhsProblem& p, const double* gN_final, double* gN0,
+                                    double* gtheta, double dt, int steps, int every,
+                                    double* ws, hipStream_t s);
+hipError_t enqueue_gtheta_reduce(const double* rows, double* out, int n, hipStream_t s);
+
 }  // namespace pbe

@@ -159,3 +159,133 @@ def rhs_on_device(N, x, agg=None, brk=None, t_q=None, bw_q=None):
                   "launch_breakage_rhs")
     hip.synchronize()
     return drhs.download()
+
+
+# ------------------------------------------------------------ integration --
+METHOD_RK4 = 0
+METHOD_EULER = 1
+
+
+class Integrator:
+    """Device RK4 / Euler over R_agg + R_brk (pbe_integrate and the hipGraph
+    entry points).  Holds device copies of x, t_q, bw_q and the workspace;
+    N is a DeviceArray advanced in place."""
+
+    def __init__(self, x, agg=None, brk=None, t_q=None, bw_q=None, method=METHOD_RK4):
+        import numpy as np
+        L = library()
+        self.n = len(x)
+        self.x = hip.DeviceArray.from_host(x)
+        self.agg = agg_params(self.n, agg) if agg is not None else None
+        self.brk = None
+        self.t_q = self.bw_q = None
+        if brk is not None:
+            t_q, bw_q = np.atleast_1d(t_q), np.atleast_1d(bw_q)
+            self.t_q = hip.DeviceArray.from_host(t_q)
+            self.bw_q = hip.DeviceArray.from_host(bw_q)
+            self.brk = brk_params(self.n, len(t_q), brk)
+        self.method = method
+        self.ws_bytes = L.pbe_integrator_workspace_bytes(self.n, method)
+        self.ws = hip.DeviceArray(self.ws_bytes // 8)
+        self.graphs = []
+
+    def _args(self):
+        return (_ptr(self.x), _ptr(self.t_q), _ptr(self.bw_q),
+                ctypes.byref(self.agg) if self.agg is not None else None,
+                ctypes.byref(self.brk) if self.brk is not None else None)
+
+    def step(self, N, dt, steps=1, stream=None):
+        x, tq, bw, a, b = self._args()
+        return library().pbe_integrate(_ptr(N), x, tq, bw, a, b, float(dt), int(steps),
+                                       self.method, self.ws.ptr, self.ws_bytes,
+                                       _stream(stream))
+
+    def graph(self, N, dt, steps_per_launch):
+        """Capture `steps_per_launch` steps into a hipGraph; returns a Graph."""
+        x, tq, bw, a, b = self._args()
+        g = ctypes.c_void_p()
+        hip.check(library().pbe_integrator_graph_create(
+            ctypes.byref(g), _ptr(N), x, tq, bw, a, b, float(dt), int(steps_per_launch),
+            self.method, self.ws.ptr, self.ws_bytes), "pbe_integrator_graph_create")
+        gr = Graph(g)
+        self.graphs.append(gr)
+        return gr
+
+
+class Graph:
+    def __init__(self, handle):
+        self.handle = handle
+
+    def launch(self, stream=None):
+        return library().pbe_graph_launch(self.handle, _stream(stream))
+
+    def destroy(self):
+        if self.handle is not None and self.handle.value:
+            library().pbe_graph_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:  # noqa: BLE001
+            pass
+
+
+# ---------------------------------------------------------------- batched --
+class BatchInstance(ctypes.Structure):
+    """pbe_batch_instance (include/pbe_hip/pbe_hip.h)."""
+    _fields_ = [("n", ctypes.c_int), ("offset", ctypes.c_int), ("log_x0", ctypes.c_double),
+                ("inv_log_r", ctypes.c_double), ("beta0", ctypes.c_double),
+                ("beta_br", ctypes.c_double), ("beta_sh", ctypes.c_double),
+                ("S0", ctypes.c_double), ("v_ref", ctypes.c_double),
+                ("alpha", ctypes.c_double), ("v_min", ctypes.c_double)]
+
+
+class BatchPlan:
+    """A pbe_batch_plan over instances given as dicts with keys n, offset and
+    the coefficients (log_x0, inv_log_r, beta0, beta_br, beta_sh, S0, v_ref,
+    alpha, v_min).  kernel_type / selection = -1 switch a term off."""
+
+    def __init__(self, instances, kernel_type, selection, n_quad):
+        arr = (BatchInstance * len(instances))()
+        for a, d in zip(arr, instances):
+            a.n, a.offset = int(d["n"]), int(d["offset"])
+            for f in ("log_x0", "inv_log_r", "beta0", "beta_br", "beta_sh", "S0", "v_ref",
+                      "alpha", "v_min"):
+                setattr(a, f, float(d.get(f, 0.0)))
+        self.handle = ctypes.c_void_p()
+        hip.check(library().pbe_batch_plan_create(ctypes.byref(self.handle), arr,
+                                                  len(instances), int(kernel_type),
+                                                  int(selection), int(n_quad)),
+                  "pbe_batch_plan_create")
+        self.size = library().pbe_batch_plan_size(self.handle)
+        self.instances = instances
+
+    def launch(self, N, x, t_q, bw_q, rhs, stream=None):
+        return library().pbe_launch_batched_rhs(self.handle, _ptr(N), _ptr(x), _ptr(t_q),
+                                                _ptr(bw_q), _ptr(rhs), _stream(stream))
+
+    def integrate(self, N, x, t_q, bw_q, dt, steps, ws, ws_bytes, method=METHOD_RK4,
+                  stream=None):
+        return library().pbe_batch_integrate(self.handle, _ptr(N), _ptr(x), _ptr(t_q),
+                                             _ptr(bw_q), float(dt), int(steps), int(method),
+                                             _ptr(ws), int(ws_bytes), _stream(stream))
+
+    def graph(self, N, x, t_q, bw_q, dt, steps_per_launch, ws, ws_bytes, method=METHOD_RK4):
+        g = ctypes.c_void_p()
+        hip.check(library().pbe_batch_integrator_graph_create(
+            ctypes.byref(g), self.handle, _ptr(N), _ptr(x), _ptr(t_q), _ptr(bw_q), float(dt),
+            int(steps_per_launch), int(method), _ptr(ws), int(ws_bytes)),
+            "pbe_batch_integrator_graph_create")
+        return Graph(g)
+
+    def destroy(self):
+        if self.handle is not None and self.handle.value:
+            library().pbe_batch_plan_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:  # noqa: BLE001
+            pass

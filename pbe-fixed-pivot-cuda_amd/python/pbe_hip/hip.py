@@ -38,9 +38,11 @@ class HipError(RuntimeError):
 
 
 def _candidates():
+    # by soname first: if another component of the process (e.g. torch) has
+    # already loaded a HIP runtime, this binds to that same copy
     rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
-    return [os.path.join(rocm, "lib", "libamdhip64.so.7"),
-            os.path.join(rocm, "lib", "libamdhip64.so"), "libamdhip64.so.7"]
+    return ["libamdhip64.so.7", os.path.join(rocm, "lib", "libamdhip64.so.7"),
+            os.path.join(rocm, "lib", "libamdhip64.so")]
 
 
 def runtime():

@@ -186,3 +186,45 @@ def pair_units(n, n_quad):
 PAIR_FLOPS = {0: 11, 1: 12, 2: 13, 3: 16, 4: 15, 5: 21}
 BREAKAGE_UPDATE_FLOPS = 8
 BREAKAGE_PARENT_FLOPS = 2
+
+
+def config_batched(count=4096, seed=4096, n_min=64, n_max=2048):
+    """Batched workload: `count` independent instances, n log-uniform in
+    [n_min, n_max] on geometric grids r = 1.03 from x_0 = 1e-18, each with its
+    own shear rate G log-uniform in [10, 500] 1/s (beta_sh = 4/3 G,
+    beta_br = 6.73e-18), power-law breakage S0 = 1e-4 G^1.5 with exponent 1/3
+    and v_ref = x_{n/2}, symmetric-binary daughters, and its own lognormal N
+    (median pivot index uniform in [n/8, n/2], sigma_ln uniform in [0.5, 1.2],
+    total 1e12, +-5 % noise).  Returns (instances, N, x, t_q, bw_q) with N and
+    x packed back to back in instance order."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(math.log(n_min), math.log(n_max + 1), count)).astype(int)
+    ns = np.clip(ns, n_min, n_max)
+    Gs = np.exp(rng.uniform(math.log(10.0), math.log(500.0), count))
+    sig = rng.uniform(0.5, 1.2, count)
+    xs, Ns, inst = [], [], []
+    off = 0
+    for i in range(count):
+        n = int(ns[i])
+        x = geometric_grid(n, 1e-18, 1.03)
+        med = int(rng.integers(max(1, n // 8), max(2, n // 2)))
+        N = lognormal_ic(x, x[med], float(sig[i]), 1e12, noise=0.05,
+                         seed=int(rng.integers(1 << 30)))
+        log_x0, inv_log_r = grid_lookup(x)
+        G = float(Gs[i])
+        inst.append(dict(n=n, offset=off, log_x0=log_x0, inv_log_r=inv_log_r, beta0=0.0,
+                         beta_br=6.73e-18, beta_sh=4.0 / 3.0 * G, S0=1e-4 * G ** 1.5,
+                         v_ref=float(x[n // 2]), alpha=1.0 / 3.0, v_min=0.0, G=G))
+        xs.append(x)
+        Ns.append(N)
+        off += n
+    t_q, bw_q = daughter_symmetric()
+    return inst, np.concatenate(Ns), np.concatenate(xs), t_q, bw_q
+
+
+def instance_params(d):
+    """(agg, brk) parameter dicts of one batched instance for the plain calls."""
+    agg = dict(kernel=5, beta0=d["beta0"], beta_br=d["beta_br"], beta_sh=d["beta_sh"],
+               log_x0=d["log_x0"], inv_log_r=d["inv_log_r"])
+    brk = dict(selection=2, S0=d["S0"], v_ref=d["v_ref"], alpha=d["alpha"], v_min=d["v_min"])
+    return agg, brk

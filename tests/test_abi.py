@@ -1,0 +1,119 @@
+"""CPU tests of the C-ABI library: it loads without a GPU, exports every
+function declared in include/pbe_hip/*.h, its parameter structs have the
+reference's layout, and every argument check returns the reference's error
+code before any HIP runtime call (src/aggregation.cu:23-24,54;
+src/breakage.cu:23-24,46,74)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER_DIR = os.path.join(REPO, "include", "pbe_hip")
+
+INVALID_VALUE = 1
+INVALID_CONFIGURATION = 9
+INVALID_DEVICE_POINTER = 17
+
+
+def _declared_functions():
+    names = set()
+    for fn in os.listdir(HEADER_DIR):
+        if fn.endswith(".h"):
+            text = open(os.path.join(HEADER_DIR, fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(pbe_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_exports_every_declared_symbol(pbe):
+    lib = pbe.library()
+    names = _declared_functions()
+    assert "pbe_launch_aggregation_rhs" in names and "pbe_launch_breakage_rhs" in names
+    for name in names:
+        assert hasattr(lib, name), f"{name} declared but not exported"
+
+
+def test_param_struct_layouts(pbe):
+    A, B = pbe.AggregationParams, pbe.BreakageParams
+    assert ctypes.sizeof(A) == 64 and ctypes.sizeof(B) == 56
+    assert [getattr(A, f).offset for f, _ in A._fields_] == [0, 8, 16, 24, 32, 40, 48, 56]
+    assert [getattr(B, f).offset for f, _ in B._fields_] == [0, 4, 8, 16, 24, 32, 40, 48]
+    a = A()
+    assert (a.n, a.log_x0, a.inv_log_r, a.kernel_type, a.beta0, a.beta_br, a.beta_sh,
+            a.block_size) == (0, 0.0, 0.0, 0, 1.0, 0.0, 0.0, 256)
+    b = B()
+    assert (b.n, b.n_quad, b.selection, b.S0, b.v_ref, b.alpha, b.v_min,
+            b.block_size) == (0, 0, 0, 1.0, 1.0, 1.0, 0.0, 256)
+
+
+FAKE = 0x1000  # never dereferenced: every case below fails validation first
+
+
+@pytest.mark.parametrize("kw,ptrs,code", [
+    (dict(n=0), (FAKE, FAKE, FAKE), INVALID_VALUE),
+    (dict(n=-5), (FAKE, FAKE, FAKE), INVALID_VALUE),
+    (dict(n=0), (None, None, None), INVALID_VALUE),            # n is checked first
+    (dict(n=10), (None, FAKE, FAKE), INVALID_DEVICE_POINTER),
+    (dict(n=10), (FAKE, None, FAKE), INVALID_DEVICE_POINTER),
+    (dict(n=10), (FAKE, FAKE, None), INVALID_DEVICE_POINTER),
+    (dict(n=10, kernel_type=6), (FAKE, FAKE, FAKE), INVALID_VALUE),
+    (dict(n=10, kernel_type=-1), (FAKE, FAKE, FAKE), INVALID_VALUE),
+    (dict(n=10, kernel_type=6), (None, FAKE, FAKE), INVALID_DEVICE_POINTER),  # ptr before enum
+    (dict(n=10, block_size=2048), (FAKE, FAKE, FAKE), INVALID_CONFIGURATION),
+])
+def test_aggregation_argument_errors(pbe, kw, ptrs, code):
+    p = pbe.AggregationParams(**kw)
+    assert pbe.launch_aggregation_rhs(*ptrs, p) == code
+
+
+@pytest.mark.parametrize("kw,ptrs,code", [
+    (dict(n=0, n_quad=1), (FAKE,) * 5, INVALID_VALUE),
+    (dict(n=10, n_quad=0), (FAKE,) * 5, INVALID_VALUE),
+    (dict(n=10, n_quad=-1), (None,) * 5, INVALID_VALUE),
+    (dict(n=10, n_quad=1), (None, FAKE, FAKE, FAKE, FAKE), INVALID_DEVICE_POINTER),
+    (dict(n=10, n_quad=1), (FAKE, FAKE, None, FAKE, FAKE), INVALID_DEVICE_POINTER),
+    (dict(n=10, n_quad=1), (FAKE, FAKE, FAKE, None, FAKE), INVALID_DEVICE_POINTER),
+    (dict(n=10, n_quad=1), (FAKE, FAKE, FAKE, FAKE, None), INVALID_DEVICE_POINTER),
+    (dict(n=10, n_quad=1, selection=4), (FAKE,) * 5, INVALID_VALUE),
+    (dict(n=10, n_quad=1, block_size=0), (FAKE,) * 5, INVALID_VALUE),
+    (dict(n=10, n_quad=1, block_size=4096), (FAKE,) * 5, INVALID_CONFIGURATION),
+])
+def test_breakage_argument_errors(pbe, kw, ptrs, code):
+    p = pbe.BreakageParams(**kw)
+    assert pbe.launch_breakage_rhs(*ptrs, p) == code
+
+
+def test_null_params_pointer(pbe):
+    lib = pbe.library()
+    assert lib.pbe_launch_aggregation_rhs(FAKE, FAKE, FAKE, None, None) == INVALID_VALUE
+    assert lib.pbe_launch_breakage_rhs(FAKE, FAKE, FAKE, FAKE, FAKE, None, None) == INVALID_VALUE
+
+
+def test_version(pbe):
+    assert "gfx950" in pbe.version()
+
+
+def test_daughter_tables():
+    """Moments of the daughter tables: number sum bw and volume sum bw t."""
+    import numpy as np
+    from pbe_hip import problems as P
+    t, bw = P.daughter_symmetric()
+    assert bw.sum() == 2.0 and (bw * t).sum() == 1.0
+    t, bw = P.daughter_erosion(0.05)
+    assert bw.sum() == 2.0 and abs((bw * t).sum() - 1.0) < 1e-15
+    for m in (4, 8, 16, 32):
+        t, bw = P.daughter_uniform(m)
+        assert abs(bw.sum() - 2.0) < 1e-14 and abs((bw * t).sum() - 1.0) < 1e-14
+        assert np.all((t > 0) & (t < 1))
+    for q in (0.0, 1.0, 2.0, 3.0):
+        for m in (8, 16, 32):
+            t, bw = P.daughter_power_law(q, m)
+            # b = (q+2)/v' (v/v')^q integrates to nu = (q+2)/(q+1) fragments and
+            # conserves volume; GL with m/2 points per panel is exact for
+            # integer q < m - 1
+            assert abs(bw.sum() - (q + 2) / (q + 1)) < 1e-13
+            assert abs((bw * t).sum() - 1.0) < 1e-13
+    with pytest.raises(ValueError):
+        P.daughter_power_law(2.0, 7)

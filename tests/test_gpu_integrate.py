@@ -1,0 +1,94 @@
+"""Device time integration (pbe_integrate) and its hipGraph against the
+oracle's RK4 (the reference harness arithmetic,
+examples/common/pbe_examples_utils.hpp:416-457) and the Smoluchowski closed
+form."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_smoluchowski_rk4_closed_form(pbe, oracle_mod):
+    """Constant kernel, n = 256, x_i = 1e-18 2^(i/4), dt = 1 s, 1000 steps:
+    M0(t) = M0(0)/(1 + beta0 M0(0) t/2) to RK4 accuracy (4.2e-7 expected)."""
+    from pbe_hip import hip, problems as P
+    x, N0, agg = P.config_smoluchowski(256)
+    it = pbe.Integrator(x, agg=agg)
+    dN = hip.DeviceArray.from_host(N0)
+    hip.check(it.step(dN, 1.0, 1000))
+    hip.synchronize()
+    N = dN.download()
+    M00 = N0.sum()
+    exact = M00 / (1.0 + 1e-12 * M00 * 1000.0 / 2.0)
+    assert abs(N.sum() - exact) / exact <= 1e-6
+    ref = oracle_mod.rk4(N0, x, 1.0, 1000, agg=agg)
+    scale = np.abs(ref).max()
+    err = np.abs(N - ref).max() / scale
+    print("rk4 1000 steps, max |gpu-oracle|/max|N| =", err)
+    assert err <= 1e-11
+
+
+def test_flocculation_rk4_vs_oracle(pbe, oracle_mod):
+    from pbe_hip import hip, problems as P
+    x, N0, agg, brk, t_q, bw_q = P.config_flocculation(512, daughter="uniform", n_quad=8)
+    brk = dict(brk, S0=0.05)
+    it = pbe.Integrator(x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    dN = hip.DeviceArray.from_host(N0)
+    hip.check(it.step(dN, 0.5, 40))
+    hip.synchronize()
+    N = dN.download()
+    ref = oracle_mod.rk4(N0, x, 0.5, 40, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    err = np.abs(N - ref).max() / np.abs(ref).max()
+    print("agg+brk rk4 40 steps:", err)
+    assert err <= 1e-11
+
+
+def test_euler_vs_oracle_one_step(pbe, oracle_mod):
+    from pbe_hip import hip, problems as P
+    x, N0, agg, brk, t_q, bw_q = P.config_flocculation(300)
+    it = pbe.Integrator(x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q, method=pbe.METHOD_EULER)
+    dN = hip.DeviceArray.from_host(N0)
+    hip.check(it.step(dN, 0.25, 1))
+    hip.synchronize()
+    ra, Aa, _ = oracle_mod.aggregation_rhs(N0, x, **agg)
+    rb, Ab, _ = oracle_mod.breakage_rhs(N0, x, t_q, bw_q, **brk)
+    expect = N0 + 0.25 * (ra + rb)
+    tol = 1e-12 * 0.25 * (Aa + Ab) + 1e-15 * np.abs(N0)
+    assert np.all(np.abs(dN.download() - expect) <= tol + 1e-300)
+
+
+def test_graph_replays_match_eager(pbe):
+    """The captured RK4 graph, replayed several times with eager work and
+    synchronisations in between, tracks the eager integration after every
+    replay (no memset node is recorded)."""
+    from pbe_hip import hip, problems as P
+    x, N0, agg, brk, t_q, bw_q = P.config_flocculation(1024)
+    brk = dict(brk, S0=0.05)
+    it_g = pbe.Integrator(x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    it_e = pbe.Integrator(x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    dNg = hip.DeviceArray.from_host(N0)
+    dNe = hip.DeviceArray.from_host(N0)
+    g = it_g.graph(dNg, 0.5, 5)
+    s = hip.Stream()
+    for rep in range(4):
+        hip.check(g.launch(s))
+        s.synchronize()
+        hip.check(it_e.step(dNe, 0.5, 5))
+        hip.synchronize()
+        a, b = dNg.download(), dNe.download()
+        err = np.abs(a - b).max() / np.abs(b).max()
+        assert err <= 1e-12, (rep, err)
+        assert not np.array_equal(a, N0)
+    g.destroy()
+
+
+def test_integrator_argument_errors(pbe):
+    from pbe_hip import hip
+    L = pbe.library()
+    p = pbe.AggregationParams(n=10)
+    assert L.pbe_integrate(1, 1, None, None, None, None, 1.0, 1, 0, 1, 10 ** 6, None) == 1
+    import ctypes
+    assert L.pbe_integrate(1, None, None, None, ctypes.byref(p), None, 1.0, 1, 0, 1, 10 ** 6,
+                           None) == 17
+    assert L.pbe_integrate(1, 1, None, None, ctypes.byref(p), None, 1.0, 1, 0, 1, 8,
+                           None) == 1
