@@ -38,7 +38,82 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=2.0,
                     help="wall-clock budget of the CPU baseline sample")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the batched and adjoint comparisons")
     return ap.parse_args()
+
+
+def batched_form(hip, P):
+    """4096 instances (n log-uniform in [64, 2048]): one batched launch vs a
+    loop of plain per-instance calls, timed by the C++ helper on one stream."""
+    import struct
+    import subprocess
+    import tempfile
+    inst, N, x, t_q, bw_q = P.config_batched(4096, seed=4096)
+    exe = os.path.join(REPO, "pbe-fixed-pivot-cuda_amd", "bench", "batched_loop")
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "batch.bin")
+        with open(path, "wb") as f:
+            f.write(struct.pack("iii", len(inst), len(N), len(t_q)))
+            for d in inst:
+                f.write(struct.pack("ii9d", d["n"], d["offset"], d["log_x0"], d["inv_log_r"],
+                                    d["beta0"], d["beta_br"], d["beta_sh"], d["S0"], d["v_ref"],
+                                    d["alpha"], d["v_min"]))
+            for a in (N, x, t_q, bw_q):
+                f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+        r = subprocess.run([exe, path, "3"], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                           text=True, timeout=240)
+        if r.returncode != 0:
+            return {"error": r.stderr[-500:]}
+        out = json.loads(r.stdout.strip().splitlines()[-1])
+    pairs = sum(d["n"] * (d["n"] + 1) // 2 for d in inst)
+    out["pairs_per_eval"] = pairs
+    out["batched_pair_updates_per_s"] = (pairs + len(N)) / (out["ms_batched"] * 1e-3)
+    out["workload"] = ("4096 instances, n log-uniform [64, 2048], r = 1.03, own G in [10, 500], "
+                       "Brownian+shear + power-law breakage, binary daughters")
+    return out
+
+
+def adjoint_form(pbe, hip, P):
+    """n = 1024, 2000 RK4 steps of 0.5 s: gradient (checkpointed adjoint
+    forward + reverse) vs a plain forward solve, both timed with HIP events."""
+    x, N0, agg, brk, t_q, bw_q = P.config_adjoint(1024)
+    dt, steps, every = 0.5, 2000, 50
+    s = hip.Stream()
+    it = pbe.Integrator(x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    dN = hip.DeviceArray.from_host(N0)
+    hip.check(it.step(dN, dt, 5, s))
+    s.synchronize()
+    dN.upload(N0)
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(s)
+    hip.check(it.step(dN, dt, steps, s))
+    e1.record(s)
+    s.synchronize()
+    t_fwd = e0.elapsed_ms(e1)
+    adj = pbe.Adjoint(x, dt, steps, every, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    dA = hip.DeviceArray.from_host(N0)
+    g0, g1, g2 = hip.Event(), hip.Event(), hip.Event()
+    g0.record(s)
+    hip.check(adj.forward(dA, s))
+    g1.record(s)
+    s.synchronize()
+    Nf = dA.download()
+    logT = np.log(np.maximum(Nf * 1.1, 1e-26))
+    mask = (Nf > 1e-8 * Nf.max()).astype(float)
+    L, gfin = P.log_misfit(Nf, logT, mask, 1e-26)
+    dg, gN0, gth = hip.DeviceArray.from_host(gfin), hip.DeviceArray(len(N0)), hip.DeviceArray(5)
+    h0 = hip.Event()
+    h0.record(s)
+    hip.check(adj.backward(dg, gN0, gth, s))
+    g2.record(s)
+    s.synchronize()
+    t_afwd, t_abwd = g0.elapsed_ms(g1), h0.elapsed_ms(g2)
+    return {"n": 1024, "steps": steps, "checkpoint_every": every,
+            "forward_solve_ms": t_fwd, "adjoint_forward_ms": t_afwd,
+            "adjoint_backward_ms": t_abwd,
+            "gradient_over_forward": (t_afwd + t_abwd) / t_fwd,
+            "workload": "config_adjoint: Brownian+shear + power-law breakage, r = 1.03"}
 
 
 def cpu_baseline(N, x, agg, brk, t_q, bw_q, budget_s):
@@ -179,6 +254,8 @@ def main():
         "ms_per_step_without_memset": ms_step_nomemset,
         "wall_s_timed_region": wall,
     }
+    if not args.no_forms:
+        out["forms"] = {"batched": batched_form(hip, P), "adjoint": adjoint_form(pbe_hip, hip, P)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, x, agg, brk, t_q, bw_q, args.cpu_seconds)
     print(json.dumps(out))

@@ -88,6 +88,24 @@ hipError_t pbe_launch_aggregation_rhs(const double* N, const double* x,
                                       const pbe_aggregation_params* params,
                                       hipStream_t stream);
 
+/* Launch options of the aggregation tiles (no counterpart in the reference,
+ * whose only knob is the compile-time PBE_TILE, src/aggregation_kernels.cuh:
+ * 30-33).  tile_u = columns per lane: the pair tiles are 64*tile_u square;
+ * 0 picks the library's autotuned choice for (n, kernel_type). */
+typedef struct pbe_agg_options {
+    int tile_u;   /* 0, 1, 2, 4 or 8 */
+} pbe_agg_options;
+
+/* pbe_launch_aggregation_rhs with explicit launch options (same results). */
+hipError_t pbe_launch_aggregation_rhs_ex(const double* N, const double* x,
+                                         double* rhs,
+                                         const pbe_aggregation_params* params,
+                                         const pbe_agg_options* options,
+                                         hipStream_t stream);
+
+/* The tile_u that tile_u = 0 resolves to for (n, kernel_type). */
+int pbe_aggregation_default_tile(int n, int kernel_type);
+
 /*
  * rhs[0..n) += fixed-pivot breakage source (death and quadrature birth).
  * Replaces pbe_cuda::launch_breakage_rhs
@@ -233,6 +251,25 @@ hipError_t pbe_adjoint_rk4_backward(const double* gN_final, double* gN0,
                                     int steps, int checkpoint_every,
                                     void* workspace, size_t workspace_bytes,
                                     hipStream_t stream);
+
+/* Batched VJP and RK4 adjoint over a plan (workspace sized for
+ * pbe_batch_plan_size(plan) bins).  The instances' coefficients are fitted
+ * jointly: gtheta slots 0-3 hold sum over instances of
+ * coef_i dL/dcoef_i = dL/d ln(c) for a common scale c of beta0, beta_br,
+ * beta_sh and S0; slot 4 holds dL/dalpha. */
+hipError_t pbe_batch_rhs_vjp(pbe_batch_plan plan, const double* N, const double* x,
+                             const double* t_q, const double* bw_q, const double* lam,
+                             double* gN, double* gtheta, void* workspace,
+                             size_t workspace_bytes, hipStream_t stream);
+hipError_t pbe_batch_adjoint_rk4_forward(pbe_batch_plan plan, double* N, const double* x,
+                                         const double* t_q, const double* bw_q, double dt,
+                                         int steps, int checkpoint_every, void* workspace,
+                                         size_t workspace_bytes, hipStream_t stream);
+hipError_t pbe_batch_adjoint_rk4_backward(pbe_batch_plan plan, const double* gN_final,
+                                          double* gN0, double* gtheta, const double* x,
+                                          const double* t_q, const double* bw_q, double dt,
+                                          int steps, int checkpoint_every, void* workspace,
+                                          size_t workspace_bytes, hipStream_t stream);
 
 /* Library version string, e.g. "pbe_hip 0.1.0 gfx950". */
 const char* pbe_version(void);

@@ -1,0 +1,95 @@
+"""Pick the aggregation pair-tile shape per (n, kernel) by measurement.
+
+    python pbe-fixed-pivot-cuda_amd/autotune.py --out tile_table.inc [--json results.json]
+
+For each size bucket and collision kernel, every tile shape (64*U square
+tiles, U = 1, 2, 4, 8 columns per lane) is timed with HIP events over
+back-to-back launches on the flocculation workload (lognormal N, geometric
+grid r = 1.02), and the fastest U is written as a rule {n_max, kernel, U} in
+the C include that the library compiles in (csrc/tile_table.inc).  The table
+is read-only data built offline: the library holds no tuning state at run
+time.
+"""
+import argparse
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "python"))
+
+SIZES = [256, 512, 1024, 2048, 4096, 8192, 16384]
+KERNELS = [0, 1, 2, 3, 4, 5]
+TILES = [1, 2, 4, 8]
+
+
+def time_launch(pbe, hip, N, x, agg, u, reps):
+    import ctypes
+    n = len(N)
+    dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
+    drhs = hip.DeviceArray(n)
+    drhs.zero()
+    p = pbe.agg_params(n, agg)
+    opt = (ctypes.c_int * 1)(u)
+    s = hip.Stream()
+    L = pbe.library()
+
+    def launch():
+        hip.check(L.pbe_launch_aggregation_rhs_ex(dN.ptr, dx.ptr, drhs.ptr, ctypes.byref(p),
+                                                  ctypes.cast(opt, ctypes.c_void_p),
+                                                  s.handle), f"tile {u}")
+    for _ in range(3):
+        launch()
+    s.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(s)
+    for _ in range(reps):
+        launch()
+    e1.record(s)
+    s.synchronize()
+    return e0.elapsed_ms(e1) / reps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=os.path.join(HERE, "csrc", "tile_table.inc"))
+    ap.add_argument("--json", default=None)
+    ap.add_argument("--sizes", default=None)
+    args = ap.parse_args()
+    import pbe_hip as pbe
+    from pbe_hip import hip, problems as P
+    sizes = [int(s) for s in args.sizes.split(",")] if args.sizes else SIZES
+    results = {}
+    best = {}
+    for n in sizes:
+        x, N, agg0, _, _, _ = P.config_flocculation(n)
+        for k in KERNELS:
+            agg = dict(agg0, kernel=k, beta0=1e-17 / (x[n // 4] ** (2 if k == 2 else 1 if k == 1 else 0)))
+            reps = max(5, min(200, int(4e9 / (n * n))))
+            t = {u: time_launch(pbe, hip, N, x, agg, u, reps) for u in TILES}
+            results[f"{n}/{k}"] = t
+            best[(n, k)] = min(t, key=t.get)
+            print(f"n={n:6d} kernel={k} " + " ".join(f"U{u}:{t[u] * 1e3:8.1f}us" for u in TILES)
+                  + f"  -> U={best[(n, k)]}", flush=True)
+    lines = ["// {n_max, kernel_type (-1 = any), columns per lane}; first match wins.",
+             "// Generated by pbe-fixed-pivot-cuda_amd/autotune.py on an MI355X (gfx950)."]
+    for k in KERNELS:
+        prev = None
+        for i, n in enumerate(sizes):
+            u = best[(n, k)]
+            nmax = (sizes[i] + sizes[i + 1]) // 2 if i + 1 < len(sizes) else (1 << 30)
+            if prev is not None and prev[1] == u:
+                lines[-1] = f"{{{nmax}, {k}, {u}}},"
+            else:
+                lines.append(f"{{{nmax}, {k}, {u}}},")
+            prev = (nmax, u)
+    with open(args.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    if args.json:
+        with open(args.json, "w") as f:
+            json.dump(results, f, indent=1)
+    print("wrote", args.out)
+
+
+if __name__ == "__main__":
+    main()

@@ -73,9 +73,20 @@ def build_cpp_tests():
     return out
 
 
+def build_bench_helpers():
+    """bench/batched_loop.cpp: batched launch vs a loop of plain calls."""
+    src = os.path.join(HERE, "bench", "batched_loop.cpp")
+    out = os.path.join(HERE, "bench", "batched_loop")
+    _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", src, "-o", out + ".tmp",
+          f"-L{LIBDIR}", "-lpbe_hip", "-Wl,-rpath,$ORIGIN/../lib"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build(verbose=False):
     lib = build_library(verbose)
     exe = build_cpp_tests()
+    build_bench_helpers()
     return lib, exe
 
 

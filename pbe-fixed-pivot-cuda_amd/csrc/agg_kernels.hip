@@ -85,6 +85,9 @@ hipError_t launch_aggregation(const double* N, const double* x, double* rhs,
     args.log_x0 = p.log_x0;
     args.inv_log_r = p.inv_log_r;
     switch (tile_u) {
+    case 1:
+        args.nb = (p.n + AggShape<1>::T - 1) / AggShape<1>::T;
+        return launch_agg_u<1>(args, p.kernel_type, stream);
     case 2:
         args.nb = (p.n + AggShape<2>::T - 1) / AggShape<2>::T;
         return launch_agg_u<2>(args, p.kernel_type, stream);

@@ -45,37 +45,15 @@ struct BirthSink {
     }
 };
 
-// Birth of one near pair: clip rules of src/aggregation_kernels.cuh:191-195
-// (upper clip tested first), otherwise the exact bracket and the K-R split
-// of :196-200 and :227-231.
-template <int WIN>
-__device__ __forceinline__ void near_birth(const AggArgs& a, const BirthSink<WIN>& s,
-                                        double v, double rate, int k,
-                                        double xfirst, double xlast)
-{
-    const int n = a.n;
-    if (v >= xlast) {
-        s.put(n - 1, rate);
-    } else if (v <= xfirst) {
-        s.put(0, rate);
-    } else {
-        const int hi = (a.inv_log_r != 0.0)
-            ? bracket_geometric(a.x, n, v, a.log_x0, a.inv_log_r)
-            : bracket_search(a.x, k, n - 1, v);
-        const int lo = hi - 1;
-        const double xl = a.x[lo], xh = a.x[hi];
-        const double w = (v - xl) / (xh - xl);
-        s.put(lo, (1.0 - w) * rate);
-        s.put(hi, w * rate);
-    }
-}
-
 template <int KT, int U>
 struct AggSmem {
     static constexpr int R = Kern<KT>::R;
     static constexpr int RS = (R + 2) & ~1;   // row record: R factors + x_j, 16-B aligned
     double rowbuf[4][AggShape<U>::RW * RS];
     double colwin[AggShape<U>::WIN];
+    double xw[AggShape<U>::WIN + 1];    // pivots of the window bins
+    double idw[AggShape<U>::WIN + 1];   // 1/(x_{i+1} - x_i) of the window bins
+    double rowwin[AggShape<U>::T];      // row deaths of an off-diagonal tile
     double red[4][R];
 };
 
@@ -114,6 +92,12 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
     const int n = a.n;
 
     for (int i = tid; i < WIN; i += 256) colwin[i] = 0.0;
+    for (int i = tid; i < AggShape<U>::T; i += 256) sm.rowwin[i] = 0.0;
+    for (int i = tid; i <= WIN; i += 256) {
+        const int b = k0 + i;
+        sm.xw[i] = b < n ? a.x[b] : __builtin_inf();
+        sm.idw[i] = b < n - 1 ? 1.0 / (a.x[b + 1] - a.x[b]) : 0.0;
+    }
 
     // ---- row factors of this wave's rows -> LDS ----
     const int rw0 = j0 + wv * RW;
@@ -180,40 +164,51 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
             idx[u] = 0.0;
         }
     }
-    const double xfirst = a.x[0];
     const double xlast = a.x[n - 1];
     __syncthreads();
 
-    // ---- death of this row block (diagonal tiles only) ----
-    if (diag) {
-        double s[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) s[r] = 0.0;
-        for (int k = tid; k < n; k += 256) {
-            const double xv = a.x[k];
-            const double Nk = a.N[k];
-            double cb, icb, f[R];
-            pivot_roots<KT>(xv, cb, icb);
-            K::col(f, xv, cb, icb);
-#pragma unroll
-            for (int r = 0; r < R; ++r) s[r] = fma(Nk, f[r], s[r]);
-        }
+    // ---- death, separable within the tile ----
+    // Pairs (j < k) of this tile remove sum_k rate_jk from row j and
+    // sum_j rate_jk from column k; with the tile's column sums
+    // TK_r = sum_k B_r(k) and row sums TJ_r = sum_j A_r(j) these are
+    // sum_r A_r(j) TK_r and sum_r B_r(k) TJ_r.  On a diagonal tile (rows ==
+    // columns) bin i loses sum_{k in block} beta_ik N_i N_k = sum_r A_r(i) TK_r,
+    // which also covers the diagonal pair (2 x rate/2).  Summed over all tiles
+    // this is the full death N_i sum_k beta_ik N_k of the reference.
+    {
+        double tk[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            s[r] = wave_sum(s[r]);
-            if (lane == 0) red[wv][r] = s[r];
+            double sacc = 0.0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) sacc += Bc[u][r];
+            tk[r] = wave_sum(sacc);
+            if (lane == 0) red[wv][r] = tj[r];
         }
         __syncthreads();
-        double Sg[R];
+        double TJ[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) Sg[r] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
+        for (int r = 0; r < R; ++r) TJ[r] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
         for (int rr = lane; rr < RW; rr += kWave) {
             const int j = rw0 + rr;
             if (j < n) {
                 double dth = 0.0;
 #pragma unroll
-                for (int r = 0; r < R; ++r) dth = fma(myrows[rr * RS + r], Sg[r], dth);
-                lds_add(&colwin[j - k0], -dth);
+                for (int r = 0; r < R; ++r) dth = fma(myrows[rr * RS + r], tk[r], dth);
+                if (diag) lds_add(&colwin[j - k0], -dth);
+                else sm.rowwin[j - j0] -= dth;   // rows of this wave only: no race
+            }
+        }
+        if (!diag && wv == 0) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = kc[u];
+                if (k < n) {
+                    double dth = 0.0;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) dth = fma(Bc[u][r], TJ[r], dth);
+                    lds_add(&colwin[k - k0], -dth);
+                }
             }
         }
     }
@@ -267,10 +262,28 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
                 }
             }
         } else {
+            // Cursor sweep.  Rows run upward, so for each column the event
+            // volume v = x_j + x_k and its exact bracket (cur, cur+1) only
+            // move up: a lane keeps the bracket and the running sums
+            // sum rate and sum w*rate of the pairs landing in it, and emits
+            // them when the bracket moves on.  Far pairs are the case cur = k.
             const BirthSink<WIN> sink{colwin, a.rhs, k0};
-            double accr[U], accw[U];
+            auto xat = [&](int i) { return (i - k0 <= WIN) ? sm.xw[i - k0] : a.x[i]; };
+            auto idat = [&](int i) {
+                return (i - k0 <= WIN) ? sm.idw[i - k0] : 1.0 / (a.x[i + 1] - a.x[i]);
+            };
+            int cur[U];
+            double xc[U], xc1[U], idc[U], accr[U], accw[U], accc[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) { accr[u] = 0.0; accw[u] = 0.0; }
+            for (int u = 0; u < U; ++u) {
+                cur[u] = kc[u];
+                xc[u] = xk[u];
+                xc1[u] = xk1[u];
+                idc[u] = idx[u];
+                accr[u] = 0.0;
+                accw[u] = 0.0;
+                accc[u] = 0.0;
+            }
             for (int jj = 0; jj < nrows; ++jj) {
                 const int j = rw0 + jj;
                 const double* rp = myrows + jj * RS;
@@ -287,32 +300,59 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
                         for (int r = 1; r < R; ++r) pr = fma(av[r], Bc[u][r], pr);
                         if (j == k) pr *= 0.5;
                         const double v = xj + xk[u];
-                        if (v < xk1[u]) {
-                            accr[u] += pr;
-                            accw[u] = fma((v - xk[u]) * idx[u], pr, accw[u]);
+                        if (v >= xlast) {
+                            // upper clip (src/aggregation_kernels.cuh:191-192)
+                            accc[u] += pr;
                         } else {
-                            near_birth<WIN>(a, sink, v, pr, k, xfirst, xlast);
+                            if (v >= xc1[u]) {
+                                if (accr[u] != 0.0 || accw[u] != 0.0) {
+                                    sink.put(cur[u], accr[u] - accw[u]);
+                                    sink.put(cur[u] + 1, accw[u]);
+                                }
+                                int c = cur[u] + 1;
+                                if (xat(c + 1) <= v) {
+                                    const int hi = (a.inv_log_r != 0.0)
+                                        ? bracket_geometric(a.x, n, v, a.log_x0, a.inv_log_r)
+                                        : bracket_search(a.x, c, n - 1, v);
+                                    c = hi - 1;
+                                }
+                                cur[u] = c;
+                                xc[u] = xat(c);
+                                xc1[u] = xat(c + 1);
+                                idc[u] = idat(c);
+                                accr[u] = 0.0;
+                                accw[u] = 0.0;
+                            }
+                            accr[u] += pr;
+                            accw[u] = fma((v - xc[u]) * idc[u], pr, accw[u]);
                         }
                     }
                 }
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int k = kc[u];
-                if (k < n - 1) {
-                    lds_add(&colwin[k - k0], accr[u] - accw[u]);
-                    lds_add(&colwin[k + 1 - k0], accw[u]);
+                if (accr[u] != 0.0 || accw[u] != 0.0) {
+                    sink.put(cur[u], accr[u] - accw[u]);
+                    sink.put(cur[u] + 1, accw[u]);
                 }
+                if (accc[u] != 0.0) sink.put(n - 1, accc[u]);
             }
         }
     }
     __syncthreads();
 
-    // ---- flush the window ----
+    // ---- flush the windows ----
     for (int i = tid; i < WIN; i += 256) {
         const int k = k0 + i;
         const double v = colwin[i];
         if (k < n && v != 0.0) global_add(&a.rhs[k], v);
+    }
+    if (!diag) {
+        for (int i = tid; i < AggShape<U>::T; i += 256) {
+            const int j = j0 + i;
+            const double v = sm.rowwin[i];
+            if (j < n && v != 0.0) global_add(&a.rhs[j], v);
+        }
     }
 }
 

@@ -16,23 +16,11 @@
 #include <vector>
 
 #include "agg_tile.hpp"
+#include "batch_plan.hpp"
 #include "brk_block.hpp"
 #include "pbe_internal.hpp"
 
 namespace pbe {
-
-constexpr int kBatchU = 2;                      // 128 x 128 pair tiles
-constexpr int kBatchT = AggShape<kBatchU>::T;
-
-struct BatchInstDev {
-    int n, off, nb, pad;
-    double log_x0, inv_log_r, b0, b_br, b_sh, S0, v_ref, alpha, v_min;
-};
-
-struct BatchItem {
-    int inst;
-    int unit;       // tile index (diagonal-major) or parent block index
-};
 
 struct BatchArgs {
     const double* __restrict__ N;
@@ -87,18 +75,6 @@ void brk_batch_kernel(BatchArgs b)
 
 }  // namespace pbe
 
-struct pbe_batch_plan_s {
-    int count = 0;
-    int total = 0;              // max(offset + n) over instances
-    int kernel_type = -1;       // -1: no aggregation
-    int selection = -1;         // -1: no breakage
-    int n_quad = 0;
-    int n_agg_items = 0, n_brk_items = 0;
-    int grid = 0;
-    pbe::BatchInstDev* d_inst = nullptr;
-    pbe::BatchItem* d_agg_items = nullptr;
-    pbe::BatchItem* d_brk_items = nullptr;
-};
 
 namespace pbe {
 
@@ -169,6 +145,13 @@ hipError_t batch_plan_create(pbe_batch_plan* out, const pbe_batch_instance* hi, 
                                BatchItem{i, b}});
         }
     }
+    std::vector<Costed> rows;
+    for (int i = 0; i < count; ++i) {
+        const int blocks = (inst[i].n + kWave - 1) / kWave;
+        for (int b = 0; b < blocks; ++b)
+            rows.push_back({(double)std::min(kWave, inst[i].n - b * kWave) * inst[i].n,
+                            BatchItem{i, b}});
+    }
     auto by_cost = [](const Costed& a, const Costed& b) {
         if (a.c != b.c) return a.c > b.c;
         if (a.w.inst != b.w.inst) return a.w.inst < b.w.inst;
@@ -176,6 +159,7 @@ hipError_t batch_plan_create(pbe_batch_plan* out, const pbe_batch_instance* hi, 
     };
     std::sort(agg.begin(), agg.end(), by_cost);
     std::sort(brk.begin(), brk.end(), by_cost);
+    std::sort(rows.begin(), rows.end(), by_cost);
 
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
@@ -215,6 +199,15 @@ hipError_t batch_plan_create(pbe_batch_plan* out, const pbe_batch_instance* hi, 
             e = hipMemcpy(p->d_brk_items, items.data(), sizeof(BatchItem) * items.size(),
                           hipMemcpyHostToDevice);
     }
+    if (e == hipSuccess && !rows.empty()) {
+        items.clear();
+        for (auto& c : rows) items.push_back(c.w);
+        p->n_row_items = (int)items.size();
+        e = hipMalloc(&p->d_row_items, sizeof(BatchItem) * items.size());
+        if (e == hipSuccess)
+            e = hipMemcpy(p->d_row_items, items.data(), sizeof(BatchItem) * items.size(),
+                          hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         (void)batch_plan_destroy(p);
         return e;
@@ -227,7 +220,8 @@ hipError_t batch_plan_destroy(pbe_batch_plan p)
 {
     if (!p) return hipErrorInvalidValue;
     hipError_t e = hipSuccess;
-    for (void* q : {(void*)p->d_inst, (void*)p->d_agg_items, (void*)p->d_brk_items}) {
+    for (void* q : {(void*)p->d_inst, (void*)p->d_agg_items, (void*)p->d_brk_items,
+                    (void*)p->d_row_items}) {
         if (q) {
             hipError_t e2 = hipFree(q);
             if (e == hipSuccess) e = e2;

@@ -24,9 +24,19 @@ static_assert(offsetof(pbe_breakage_params, block_size) == 48, "layout");
 
 namespace pbe {
 
-int default_tile_u(int n, int /*kernel_type*/)
+// Columns per lane of the aggregation tiles, from the autotuned table
+// (tile_table.inc: rows {n_max, kernel_type, tile_u}, generated on an MI355X
+// by pbe-fixed-pivot-cuda_amd/autotune.py; kernel_type -1 matches any).
+struct TileRule { int n_max, kernel, u; };
+static const TileRule kTileTable[] = {
+#include "tile_table.inc"
+};
+
+int default_tile_u(int n, int kernel_type)
 {
-    return n <= 512 ? 2 : 4;
+    for (const TileRule& r : kTileTable)
+        if (n <= r.n_max && (r.kernel < 0 || r.kernel == kernel_type)) return r.u;
+    return 4;
 }
 
 }  // namespace pbe
@@ -45,6 +55,29 @@ extern "C" hipError_t pbe_launch_aggregation_rhs(const double* N, const double* 
                                            stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
+}
+
+extern "C" hipError_t pbe_launch_aggregation_rhs_ex(const double* N, const double* x,
+                                                    double* rhs,
+                                                    const pbe_aggregation_params* p,
+                                                    const pbe_agg_options* opts,
+                                                    hipStream_t stream)
+{
+    if (!p || p->n <= 0) return hipErrorInvalidValue;
+    if (!N || !x || !rhs) return hipErrorInvalidDevicePointer;
+    if (p->kernel_type < 0 || p->kernel_type > 5) return hipErrorInvalidValue;
+    if (p->block_size > 1024) return hipErrorInvalidConfiguration;
+    int u = opts ? opts->tile_u : 0;
+    if (u == 0) u = pbe::default_tile_u(p->n, p->kernel_type);
+    if (u != 1 && u != 2 && u != 4 && u != 8) return hipErrorInvalidValue;
+    hipError_t e = pbe::launch_aggregation(N, x, rhs, *p, u, stream);
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+extern "C" int pbe_aggregation_default_tile(int n, int kernel_type)
+{
+    return pbe::default_tile_u(n, kernel_type);
 }
 
 extern "C" hipError_t pbe_launch_breakage_rhs(const double* N, const double* x,
@@ -337,6 +370,63 @@ extern "C" hipError_t pbe_adjoint_rk4_backward(const double* gN_final, double* g
     pbe::RhsProblem p;
     hipError_t e = make_problem(p, x, t_q, bw_q, agg, brk);
     if (e != hipSuccess) return e;
+    if (!gN_final || !gN0 || !gtheta) return hipErrorInvalidDevicePointer;
+    if ((e = adjoint_checks(p, steps, checkpoint_every, workspace, workspace_bytes)) != hipSuccess)
+        return e;
+    return pbe::enqueue_adjoint_backward(p, gN_final, gN0, gtheta, dt, steps, checkpoint_every,
+                                         static_cast<double*>(workspace), stream);
+}
+
+// ------------------------------------------- batched VJP and RK4 adjoint ----
+extern "C" hipError_t pbe_batch_rhs_vjp(pbe_batch_plan plan, const double* N, const double* x,
+                                        const double* t_q, const double* bw_q,
+                                        const double* lam, double* gN, double* gtheta,
+                                        void* workspace, size_t workspace_bytes,
+                                        hipStream_t stream)
+{
+    pbe::RhsProblem p;
+    hipError_t e = batch_problem(p, plan, x, t_q, bw_q);
+    if (e != hipSuccess) return e;
+    if (pbe::batch_plan_needs_table(plan) && (!t_q || !bw_q)) return hipErrorInvalidDevicePointer;
+    if (!N || !lam || !gN || !gtheta || !workspace) return hipErrorInvalidDevicePointer;
+    if (workspace_bytes < pbe_vjp_workspace_bytes(p.n)) return hipErrorInvalidValue;
+    double* tab = static_cast<double*>(workspace);
+    double* rows = tab + 4 * (size_t)p.n;
+    if ((e = pbe::enqueue_vjp_table(x, tab, p.n, stream)) != hipSuccess) return e;
+    if ((e = pbe::enqueue_fill_zero(rows, 5 * p.n, stream)) != hipSuccess) return e;
+    if ((e = pbe::enqueue_vjp(p, N, lam, tab, gN, rows, stream)) != hipSuccess) return e;
+    return pbe::enqueue_gtheta_reduce(rows, gtheta, p.n, stream);
+}
+
+extern "C" hipError_t pbe_batch_adjoint_rk4_forward(pbe_batch_plan plan, double* N,
+                                                    const double* x, const double* t_q,
+                                                    const double* bw_q, double dt, int steps,
+                                                    int checkpoint_every, void* workspace,
+                                                    size_t workspace_bytes, hipStream_t stream)
+{
+    pbe::RhsProblem p;
+    hipError_t e = batch_problem(p, plan, x, t_q, bw_q);
+    if (e != hipSuccess) return e;
+    if (pbe::batch_plan_needs_table(plan) && (!t_q || !bw_q)) return hipErrorInvalidDevicePointer;
+    if (!N) return hipErrorInvalidDevicePointer;
+    if ((e = adjoint_checks(p, steps, checkpoint_every, workspace, workspace_bytes)) != hipSuccess)
+        return e;
+    return pbe::enqueue_adjoint_forward(p, N, dt, steps, checkpoint_every,
+                                        static_cast<double*>(workspace), stream);
+}
+
+extern "C" hipError_t pbe_batch_adjoint_rk4_backward(pbe_batch_plan plan,
+                                                     const double* gN_final, double* gN0,
+                                                     double* gtheta, const double* x,
+                                                     const double* t_q, const double* bw_q,
+                                                     double dt, int steps, int checkpoint_every,
+                                                     void* workspace, size_t workspace_bytes,
+                                                     hipStream_t stream)
+{
+    pbe::RhsProblem p;
+    hipError_t e = batch_problem(p, plan, x, t_q, bw_q);
+    if (e != hipSuccess) return e;
+    if (pbe::batch_plan_needs_table(plan) && (!t_q || !bw_q)) return hipErrorInvalidDevicePointer;
     if (!gN_final || !gN0 || !gtheta) return hipErrorInvalidDevicePointer;
     if ((e = adjoint_checks(p, steps, checkpoint_every, workspace, workspace_bytes)) != hipSuccess)
         return e;

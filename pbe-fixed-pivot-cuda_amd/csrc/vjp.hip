@@ -17,6 +17,9 @@
 // in a fixed order, so results are bitwise repeatable.  Per-row parameter
 // partials are accumulated into a per-row array (each element owned by one
 // lane) and reduced in a fixed order at the end.
+#include <algorithm>
+
+#include "batch_plan.hpp"
 #include "pbe_device.hpp"
 #include "pbe_internal.hpp"
 
@@ -37,6 +40,7 @@ struct VjpArgs {
     KernelCoef coef;
     int sel;                          // -1: no breakage
     double S0, v_ref, alpha, v_min;
+    int log_scale;                    // 1: slots 0-3 hold d/d ln(coefficient)
 };
 
 __global__ void vjp_table_kernel(const double* __restrict__ x, double* __restrict__ tab, int n)
@@ -135,17 +139,22 @@ __device__ __forceinline__ double selection_shape(int sel, double v, double v_re
     }
 }
 
-// KT < 0: no aggregation term.
+template <int KT> struct VjpSmem {
+    static constexpr int G = (KT >= 0) ? VjpGroups<(KT >= 0 ? KT : 0)>::G : 1;
+    double part[3][kWave][G];
+};
+
+// Rows [m0, m0 + 64) of the instance described by `a`; all 256 threads call
+// it.  KT < 0: no aggregation term.
 template <int KT>
-__global__ __launch_bounds__(256)
-void vjp_kernel(VjpArgs a)
+__device__ __forceinline__ void vjp_rows_body(const VjpArgs& a, const int m0, VjpSmem<KT>& sm)
 {
-    constexpr int G = (KT >= 0) ? VjpGroups<(KT >= 0 ? KT : 0)>::G : 1;
-    __shared__ double part[3][kWave][G];
+    constexpr int G = VjpSmem<KT>::G;
+    auto& part = sm.part;
 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int n = a.n;
-    const int m = blockIdx.x * kWave + lane;
+    const int m = m0 + lane;
     const bool row = m < n;
 
     double acc[G];
@@ -222,8 +231,9 @@ void vjp_kernel(VjpArgs a)
         using VG = VjpGroups<KT>;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            gn = fma(VG::coef(g, a.coef), acc[g], gn);
-            gr[VG::slot(g)] += 0.5 * Nm * acc[g];
+            const double cf = VG::coef(g, a.coef);
+            gn = fma(cf, acc[g], gn);
+            gr[VG::slot(g)] += (a.log_scale ? cf : 1.0) * (0.5 * Nm * acc[g]);
         }
     }
     if (a.sel >= 0) {
@@ -245,11 +255,50 @@ void vjp_kernel(VjpArgs a)
             }
             const double h = e - lm;
             gn = fma(S, h, gn);
-            gr[3] += selection_shape(a.sel, xm, a.v_ref, a.alpha, a.v_min) * Nm * h;
+            gr[3] += (a.log_scale ? a.S0 : 1.0) *
+                     (selection_shape(a.sel, xm, a.v_ref, a.alpha, a.v_min) * Nm * h);
             if (a.sel == 2) gr[4] += S * log(xm / a.v_ref) * Nm * h;
         }
     }
     a.gN[m] = gn;
+}
+
+template <int KT>
+__global__ __launch_bounds__(256)
+void vjp_kernel(VjpArgs a)
+{
+    __shared__ VjpSmem<KT> sm;
+    vjp_rows_body<KT>(a, blockIdx.x * kWave, sm);
+}
+
+// Batched VJP: items are (instance, 64-row block); slots 0-3 of the per-row
+// partials hold d/d ln(coefficient) so that their sum over instances is the
+// gradient of jointly scaled coefficients.
+template <int KT>
+__global__ __launch_bounds__(256)
+void vjp_batch_kernel(VjpArgs base, const BatchInstDev* __restrict__ inst,
+                      const BatchItem* __restrict__ items, int nitems)
+{
+    __shared__ VjpSmem<KT> sm;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const BatchItem w = items[it];
+        const BatchInstDev d = inst[w.inst];
+        VjpArgs a = base;
+        a.N = base.N + d.off;
+        a.x = base.x + d.off;
+        a.lam = base.lam + d.off;
+        a.tab = base.tab + 4 * (size_t)d.off;
+        a.gN = base.gN + d.off;
+        a.gth_rows = base.gth_rows + (size_t)kNP * d.off;
+        a.n = d.n;
+        a.coef = KernelCoef{d.b0, d.b_br, d.b_sh};
+        a.S0 = d.S0;
+        a.v_ref = d.v_ref;
+        a.alpha = d.alpha;
+        a.v_min = d.v_min;
+        vjp_rows_body<KT>(a, w.unit * kWave, sm);
+        __syncthreads();
+    }
 }
 
 hipError_t enqueue_vjp_table(const double* x, double* tab, int n, hipStream_t s)
@@ -279,6 +328,27 @@ hipError_t enqueue_vjp(const RhsProblem& p, const double* N, const double* lam,
         a.v_ref = p.brk->v_ref;
         a.alpha = p.brk->alpha;
         a.v_min = p.brk->v_min;
+    }
+    if (p.batch) {
+        const pbe_batch_plan_s* pl = p.batch;
+        a.nq = pl->selection >= 0 ? pl->n_quad : 0;
+        a.sel = pl->selection;
+        a.log_scale = 1;
+        const dim3 g((unsigned)std::min(pl->grid, pl->n_row_items)), b(256);
+        const BatchInstDev* di = pl->d_inst;
+        const BatchItem* dr = pl->d_row_items;
+        const int ni = pl->n_row_items;
+        switch (pl->kernel_type) {
+        case -1: hipLaunchKernelGGL(vjp_batch_kernel<-1>, g, b, 0, s, a, di, dr, ni); break;
+        case 0: hipLaunchKernelGGL(vjp_batch_kernel<0>, g, b, 0, s, a, di, dr, ni); break;
+        case 1: hipLaunchKernelGGL(vjp_batch_kernel<1>, g, b, 0, s, a, di, dr, ni); break;
+        case 2: hipLaunchKernelGGL(vjp_batch_kernel<2>, g, b, 0, s, a, di, dr, ni); break;
+        case 3: hipLaunchKernelGGL(vjp_batch_kernel<3>, g, b, 0, s, a, di, dr, ni); break;
+        case 4: hipLaunchKernelGGL(vjp_batch_kernel<4>, g, b, 0, s, a, di, dr, ni); break;
+        case 5: hipLaunchKernelGGL(vjp_batch_kernel<5>, g, b, 0, s, a, di, dr, ni); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
     }
     const dim3 g((unsigned)((p.n + kWave - 1) / kWave)), b(256);
     switch (p.agg ? p.agg->kernel_type : -1) {

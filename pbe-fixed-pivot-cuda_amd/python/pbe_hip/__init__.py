@@ -289,3 +289,91 @@ class BatchPlan:
             self.destroy()
         except Exception:  # noqa: BLE001
             pass
+
+
+# ---------------------------------------------------------- differentiation --
+GRAD_SLOTS = ("beta0", "beta_br", "beta_sh", "S0", "alpha")
+
+
+def rhs_vjp(N, x, lam, agg=None, brk=None, t_q=None, bw_q=None):
+    """Host convenience over pbe_rhs_vjp: returns (gN, gtheta[5]) with
+    gN = lam . dR/dN and gtheta = lam . dR/d(beta0, beta_br, beta_sh, S0, alpha)."""
+    import numpy as np
+    L = library()
+    n = len(N)
+    it = Integrator(x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    dN, dl = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(lam)
+    gN, gth = hip.DeviceArray(n), hip.DeviceArray(5)
+    wsb = L.pbe_vjp_workspace_bytes(n)
+    ws = hip.DeviceArray(wsb // 8)
+    xp, tq, bw, a, b = it._args()
+    hip.check(L.pbe_rhs_vjp(_ptr(dN), xp, tq, bw, _ptr(dl), a, b, _ptr(gN), _ptr(gth),
+                            ws.ptr, wsb, None), "pbe_rhs_vjp")
+    hip.synchronize()
+    return gN.download(), gth.download()
+
+
+class Adjoint:
+    """Checkpointed discrete RK4 adjoint (pbe_adjoint_rk4_forward/backward)
+    of `steps` steps of size dt, checkpoint every `every` steps."""
+
+    def __init__(self, x, dt, steps, every=50, agg=None, brk=None, t_q=None, bw_q=None):
+        self.it = Integrator(x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+        self.n, self.dt, self.steps, self.every = len(x), float(dt), int(steps), int(every)
+        self.ws_bytes = library().pbe_adjoint_workspace_bytes(self.n, self.steps, self.every)
+        self.ws = hip.DeviceArray(self.ws_bytes // 8)
+
+    def forward(self, N, stream=None):
+        """Advance the DeviceArray / device pointer N in place, storing checkpoints."""
+        x, tq, bw, a, b = self.it._args()
+        return library().pbe_adjoint_rk4_forward(_ptr(N), x, tq, bw, a, b, self.dt,
+                                                 self.steps, self.every, self.ws.ptr,
+                                                 self.ws_bytes, _stream(stream))
+
+    def backward(self, gN_final, gN0, gtheta, stream=None):
+        x, tq, bw, a, b = self.it._args()
+        return library().pbe_adjoint_rk4_backward(_ptr(gN_final), _ptr(gN0), _ptr(gtheta), x,
+                                                  tq, bw, a, b, self.dt, self.steps,
+                                                  self.every, self.ws.ptr, self.ws_bytes,
+                                                  _stream(stream))
+
+
+class BatchAdjoint:
+    """Checkpointed RK4 adjoint over every instance of a BatchPlan
+    (pbe_batch_adjoint_rk4_forward/backward); gtheta slots 0-3 are
+    d/d ln(common scale) of beta0, beta_br, beta_sh, S0, slot 4 d/d alpha."""
+
+    def __init__(self, plan, x, t_q, bw_q, dt, steps, every=50):
+        self.plan = plan
+        self.x = hip.DeviceArray.from_host(x)
+        self.t_q = hip.DeviceArray.from_host(t_q) if t_q is not None else None
+        self.bw_q = hip.DeviceArray.from_host(bw_q) if bw_q is not None else None
+        self.dt, self.steps, self.every = float(dt), int(steps), int(every)
+        self.ws_bytes = library().pbe_adjoint_workspace_bytes(plan.size, self.steps, self.every)
+        self.ws = hip.DeviceArray(self.ws_bytes // 8)
+
+    def forward(self, N, stream=None):
+        return library().pbe_batch_adjoint_rk4_forward(
+            self.plan.handle, _ptr(N), self.x.ptr, _ptr(self.t_q), _ptr(self.bw_q), self.dt,
+            self.steps, self.every, self.ws.ptr, self.ws_bytes, _stream(stream))
+
+    def backward(self, gN_final, gN0, gtheta, stream=None):
+        return library().pbe_batch_adjoint_rk4_backward(
+            self.plan.handle, _ptr(gN_final), _ptr(gN0), _ptr(gtheta), self.x.ptr,
+            _ptr(self.t_q), _ptr(self.bw_q), self.dt, self.steps, self.every, self.ws.ptr,
+            self.ws_bytes, _stream(stream))
+
+
+def batch_rhs_vjp(plan, N, x, t_q, bw_q, lam):
+    """Host convenience over pbe_batch_rhs_vjp: (gN, gtheta[5])."""
+    L = library()
+    n = plan.size
+    dN, dx, dl = (hip.DeviceArray.from_host(a) for a in (N, x, lam))
+    dt, db = hip.DeviceArray.from_host(t_q), hip.DeviceArray.from_host(bw_q)
+    gN, gth = hip.DeviceArray(n), hip.DeviceArray(5)
+    wsb = L.pbe_vjp_workspace_bytes(n)
+    ws = hip.DeviceArray(wsb // 8)
+    hip.check(L.pbe_batch_rhs_vjp(plan.handle, dN.ptr, dx.ptr, dt.ptr, db.ptr, dl.ptr, gN.ptr,
+                                  gth.ptr, ws.ptr, wsb, None), "pbe_batch_rhs_vjp")
+    hip.synchronize()
+    return gN.download(), gth.download()

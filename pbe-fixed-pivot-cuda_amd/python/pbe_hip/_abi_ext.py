@@ -7,6 +7,8 @@ vp, i, d, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
 
 def declare(lib):
     sig = {
+        "pbe_launch_aggregation_rhs_ex": (i, [vp, vp, vp, vp, vp, vp]),
+        "pbe_aggregation_default_tile": (i, [i, i]),
         "pbe_integrator_workspace_bytes": (sz, [i, i]),
         "pbe_integrate": (i, [vp, vp, vp, vp, vp, vp, d, i, i, vp, sz, vp]),
         "pbe_integrator_graph_create": (i, [ctypes.POINTER(vp), vp, vp, vp, vp, vp, vp, d,
@@ -25,6 +27,10 @@ def declare(lib):
         "pbe_adjoint_workspace_bytes": (sz, [i, i, i]),
         "pbe_adjoint_rk4_forward": (i, [vp, vp, vp, vp, vp, vp, d, i, i, vp, sz, vp]),
         "pbe_adjoint_rk4_backward": (i, [vp, vp, vp, vp, vp, vp, vp, vp, d, i, i, vp, sz, vp]),
+        "pbe_batch_rhs_vjp": (i, [vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, vp]),
+        "pbe_batch_adjoint_rk4_forward": (i, [vp, vp, vp, vp, vp, d, i, i, vp, sz, vp]),
+        "pbe_batch_adjoint_rk4_backward": (i, [vp, vp, vp, vp, vp, vp, vp, d, i, i, vp, sz,
+                                               vp]),
     }
     for name, (res, args) in sig.items():
         if hasattr(lib, name):

@@ -228,3 +228,78 @@ def instance_params(d):
                log_x0=d["log_x0"], inv_log_r=d["inv_log_r"])
     brk = dict(selection=2, S0=d["S0"], v_ref=d["v_ref"], alpha=d["alpha"], v_min=d["v_min"])
     return agg, brk
+
+
+def config_adjoint(n=1024, seed=5, G=1.0, total=1e4, beta_br=1e-8, median_frac=0.6):
+    """Adjoint / parameter-fit workload: geometric grid r = 1.03 from
+    x_0 = 1e-18, Brownian + shear aggregation (beta_sh = 4/3 G), power-law
+    breakage S0 = 1e-4 G^1.5 with exponent 1/3 and v_ref = x_{n/2},
+    symmetric-binary daughters, lognormal N (median x_{0.6 n}, sigma_ln = 0.8).
+
+    Scales are chosen so that explicit RK4 with dt = 0.5 s is stable on the
+    whole 13-decade grid: the fastest decay rate, the death rate of the top
+    pivot sum_k beta(x_max, x_k) N_k ~ beta_sh x_max N_tot, stays near 0.2/s
+    (limit 2.78/dt), while Brownian, shear and breakage rates at the median
+    are all ~3e-4/s, so each of the four fitted parameters moves the 1000-s
+    solution.  beta_br is scaled up from the physical 6.73e-18 m^3/s to keep
+    the Brownian term identifiable at N_tot = 1e4 m^-3."""
+    x = geometric_grid(n, 1e-18, 1.03)
+    N = lognormal_ic(x, x[int(median_frac * n)], 0.8, total, noise=0.05, seed=seed)
+    log_x0, inv_log_r = grid_lookup(x)
+    agg = dict(kernel=5, beta0=0.0, beta_br=beta_br, beta_sh=4.0 / 3.0 * G,
+               log_x0=log_x0, inv_log_r=inv_log_r)
+    brk = dict(selection=2, S0=1e-4 * G ** 1.5, v_ref=float(x[n // 2]), alpha=1.0 / 3.0,
+               v_min=0.0)
+    t_q, bw_q = daughter_symmetric()
+    return x, N, agg, brk, t_q, bw_q
+
+
+def config_batched_fit(count=256, seed=256, n_min=128, n_max=1024):
+    """Batched fit workload: `count` instances with n log-uniform in
+    [n_min, n_max], each as config_adjoint with its own shear rate G
+    log-uniform in [0.5, 2] (beta_sh = 4/3 G, S0 = 1e-4 G^1.5) and its own
+    lognormal N (median pivot index uniform in [0.5 n, 0.65 n], sigma_ln
+    uniform in [0.5, 1.2], total 1e4) -- stable under RK4 with dt = 0.5 s.
+    Returns (instances, N, x, t_q, bw_q) packed like config_batched."""
+    rng = np.random.default_rng(seed)
+    ns = np.clip(np.exp(rng.uniform(math.log(n_min), math.log(n_max + 1), count)).astype(int),
+                 n_min, n_max)
+    Gs = np.exp(rng.uniform(math.log(0.5), math.log(2.0), count))
+    xs, Ns, inst = [], [], []
+    off = 0
+    for i in range(count):
+        n = int(ns[i])
+        x = geometric_grid(n, 1e-18, 1.03)
+        med = int(rng.uniform(0.5, 0.65) * n)
+        N = lognormal_ic(x, x[med], float(rng.uniform(0.5, 1.2)), 1e4, noise=0.05,
+                         seed=int(rng.integers(1 << 30)))
+        log_x0, inv_log_r = grid_lookup(x)
+        G = float(Gs[i])
+        inst.append(dict(n=n, offset=off, log_x0=log_x0, inv_log_r=inv_log_r, beta0=0.0,
+                         beta_br=1e-8, beta_sh=4.0 / 3.0 * G, S0=1e-4 * G ** 1.5,
+                         v_ref=float(x[n // 2]), alpha=1.0 / 3.0, v_min=0.0, G=G))
+        xs.append(x)
+        Ns.append(N)
+        off += n
+    t_q, bw_q = daughter_symmetric()
+    return inst, np.concatenate(Ns), np.concatenate(xs), t_q, bw_q
+
+
+FIT_PARAMS = ("beta_br", "beta_sh", "S0", "alpha")
+
+
+def with_params(agg, brk, theta):
+    """(agg, brk) with (beta_br, beta_sh, S0, alpha) replaced by theta."""
+    a, b = dict(agg), dict(brk)
+    a["beta_br"], a["beta_sh"] = float(theta[0]), float(theta[1])
+    b["S0"], b["alpha"] = float(theta[2]), float(theta[3])
+    return a, b
+
+
+def log_misfit(N, logT, mask, eps):
+    """L = sum_i mask_i (log max(N_i, eps) - logT_i)^2 and dL/dN."""
+    Nc = np.maximum(N, eps)
+    d = np.log(Nc) - logT
+    L = float(np.sum(mask * d * d))
+    g = np.where(N > eps, 2.0 * mask * d / Nc, 0.0)
+    return L, g

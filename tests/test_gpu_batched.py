@@ -9,9 +9,12 @@ from parity import assert_parity
 pytestmark = pytest.mark.gpu
 
 
-def _setup(pbe, count, seed, n_max=2048):
+def _setup(pbe, count, seed, n_max=2048, fit=False):
     from pbe_hip import hip, problems as P
-    inst, N, x, t_q, bw_q = P.config_batched(count, seed=seed, n_max=n_max)
+    if fit:
+        inst, N, x, t_q, bw_q = P.config_batched_fit(count, seed=seed, n_max=n_max)
+    else:
+        inst, N, x, t_q, bw_q = P.config_batched(count, seed=seed, n_max=n_max)
     plan = pbe.BatchPlan(inst, kernel_type=5, selection=2, n_quad=len(t_q))
     assert plan.size == len(N)
     d = dict(N=hip.DeviceArray.from_host(N), x=hip.DeviceArray.from_host(x),
@@ -63,7 +66,7 @@ def test_batched_matches_plain_loop(pbe):
 
 def test_batched_rk4_and_graph(pbe):
     from pbe_hip import hip, problems as P
-    inst, N, x, t_q, bw_q, plan, d = _setup(pbe, 64, seed=13, n_max=1024)
+    inst, N, x, t_q, bw_q, plan, d = _setup(pbe, 64, seed=13, n_max=1024, fit=True)
     L = pbe.library()
     wsb = L.pbe_integrator_workspace_bytes(len(N), pbe.METHOD_RK4)
     ws = hip.DeviceArray(wsb // 8)

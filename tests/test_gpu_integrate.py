@@ -30,14 +30,15 @@ def test_smoluchowski_rk4_closed_form(pbe, oracle_mod):
 
 def test_flocculation_rk4_vs_oracle(pbe, oracle_mod):
     from pbe_hip import hip, problems as P
-    x, N0, agg, brk, t_q, bw_q = P.config_flocculation(512, daughter="uniform", n_quad=8)
-    brk = dict(brk, S0=0.05)
+    x, N0, agg, brk, _, _ = P.config_adjoint(512)
+    t_q, bw_q = P.daughter_uniform(8)
     it = pbe.Integrator(x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
     dN = hip.DeviceArray.from_host(N0)
     hip.check(it.step(dN, 0.5, 40))
     hip.synchronize()
     N = dN.download()
     ref = oracle_mod.rk4(N0, x, 0.5, 40, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    assert np.all(np.isfinite(N))
     err = np.abs(N - ref).max() / np.abs(ref).max()
     print("agg+brk rk4 40 steps:", err)
     assert err <= 1e-11
@@ -62,8 +63,7 @@ def test_graph_replays_match_eager(pbe):
     synchronisations in between, tracks the eager integration after every
     replay (no memset node is recorded)."""
     from pbe_hip import hip, problems as P
-    x, N0, agg, brk, t_q, bw_q = P.config_flocculation(1024)
-    brk = dict(brk, S0=0.05)
+    x, N0, agg, brk, t_q, bw_q = P.config_adjoint(1024)
     it_g = pbe.Integrator(x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
     it_e = pbe.Integrator(x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
     dNg = hip.DeviceArray.from_host(N0)
@@ -76,6 +76,7 @@ def test_graph_replays_match_eager(pbe):
         hip.check(it_e.step(dNe, 0.5, 5))
         hip.synchronize()
         a, b = dNg.download(), dNe.download()
+        assert np.all(np.isfinite(a))
         err = np.abs(a - b).max() / np.abs(b).max()
         assert err <= 1e-12, (rep, err)
         assert not np.array_equal(a, N0)

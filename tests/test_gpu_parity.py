@@ -185,3 +185,29 @@ def test_flocculation_16384_vs_port(pbe, oracle_mod):
     w = assert_parity(got, ref, A, what="n=16384")
     m = assert_m1(got, x, A, D, what="n=16384")
     print(f"n=16384: entry {w:.2e} m1 {m:.2e}")
+
+
+@pytest.mark.parametrize("u", [1, 2, 4, 8])
+@pytest.mark.parametrize("n,kernel", [(700, 5), (3000, 5), (3000, 0), (1200, 2)])
+def test_every_tile_shape(pbe, oracle_mod, u, n, kernel):
+    """Each compiled tile shape (64*U square tiles) through
+    pbe_launch_aggregation_rhs_ex gives the same parity."""
+    import ctypes
+    from pbe_hip import hip, problems as P
+    x = P.geometric_grid(n, 1e-18, 1.02)
+    N = P.lognormal_ic(x, x[n // 3], 1.5, 1e12, noise=0.05, seed=u)
+    log_x0, inv_log_r = P.grid_lookup(x)
+    agg = dict(kernel=kernel, beta0=1e-17 if kernel != 2 else 1e-17 / x[n // 3],
+               beta_br=6.73e-18, beta_sh=1e-3, log_x0=log_x0, inv_log_r=inv_log_r)
+    dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
+    drhs = hip.DeviceArray(n)
+    drhs.zero()
+    opt = (ctypes.c_int * 1)(u)
+    p = pbe.agg_params(n, agg)
+    hip.check(pbe.library().pbe_launch_aggregation_rhs_ex(
+        dN.ptr, dx.ptr, drhs.ptr, ctypes.byref(p), ctypes.cast(opt, ctypes.c_void_p), None))
+    hip.synchronize()
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg)
+    got = drhs.download()
+    assert_parity(got, ref, A, what=f"U={u} n={n}")
+    assert_m1(got, x, A, D, what=f"U={u} n={n}")
