@@ -40,7 +40,21 @@ def parse():
                     help="wall-clock budget of the CPU baseline sample")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the batched and adjoint comparisons")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run an elementwise device copy of BYTES (torch.neg, "
+                         "reads and writes BYTES each), for scaling HBM byte counters")
     return ap.parse_args()
+
+
+def copy_calibration(nbytes):
+    import torch
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": src.numel() * 8, "launches": 3,
+                      "kernel": "torch.neg (elementwise)"}))
 
 
 def batched_form(hip, P):
@@ -75,10 +89,12 @@ def batched_form(hip, P):
 
 
 def adjoint_form(pbe, hip, P):
-    """n = 1024, 2000 RK4 steps of 0.5 s: gradient (checkpointed adjoint
-    forward + reverse) vs a plain forward solve, both timed with HIP events."""
+    """n = 1024, 2000 RK4 steps of 0.5 s: gradient (adjoint forward + reverse)
+    vs a plain forward solve, all timed with HIP events.  Two adjoint modes:
+    checkpoints every 50 steps with segment recompute, and every stage state
+    kept in HBM (4 x 2000 x 1024 doubles = 66 MB) with no recompute."""
     x, N0, agg, brk, t_q, bw_q = P.config_adjoint(1024)
-    dt, steps, every = 0.5, 2000, 50
+    dt, steps = 0.5, 2000
     s = hip.Stream()
     it = pbe.Integrator(x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
     dN = hip.DeviceArray.from_host(N0)
@@ -91,29 +107,36 @@ def adjoint_form(pbe, hip, P):
     e1.record(s)
     s.synchronize()
     t_fwd = e0.elapsed_ms(e1)
-    adj = pbe.Adjoint(x, dt, steps, every, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
-    dA = hip.DeviceArray.from_host(N0)
-    g0, g1, g2 = hip.Event(), hip.Event(), hip.Event()
-    g0.record(s)
-    hip.check(adj.forward(dA, s))
-    g1.record(s)
-    s.synchronize()
-    Nf = dA.download()
-    logT = np.log(np.maximum(Nf * 1.1, 1e-26))
-    mask = (Nf > 1e-8 * Nf.max()).astype(float)
-    L, gfin = P.log_misfit(Nf, logT, mask, 1e-26)
-    dg, gN0, gth = hip.DeviceArray.from_host(gfin), hip.DeviceArray(len(N0)), hip.DeviceArray(5)
-    h0 = hip.Event()
-    h0.record(s)
-    hip.check(adj.backward(dg, gN0, gth, s))
-    g2.record(s)
-    s.synchronize()
-    t_afwd, t_abwd = g0.elapsed_ms(g1), h0.elapsed_ms(g2)
-    return {"n": 1024, "steps": steps, "checkpoint_every": every,
-            "forward_solve_ms": t_fwd, "adjoint_forward_ms": t_afwd,
-            "adjoint_backward_ms": t_abwd,
-            "gradient_over_forward": (t_afwd + t_abwd) / t_fwd,
-            "workload": "config_adjoint: Brownian+shear + power-law breakage, r = 1.03"}
+    out = {"n": 1024, "steps": steps, "forward_solve_ms": t_fwd,
+           "workload": "Brownian+shear aggregation + power-law breakage, r = 1.03, n = 1024"}
+    best = None
+    for every, name in ((50, "checkpoint_50"), (pbe.STORE_ALL_STAGES, "store_all_stages")):
+        adj = pbe.Adjoint(x, dt, steps, every, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+        dA = hip.DeviceArray.from_host(N0)
+        g0, g1, g2 = hip.Event(), hip.Event(), hip.Event()
+        g0.record(s)
+        hip.check(adj.forward(dA, s))
+        g1.record(s)
+        s.synchronize()
+        Nf = dA.download()
+        logT = np.log(np.maximum(Nf * 1.1, 1e-26))
+        mask = (Nf > 1e-8 * Nf.max()).astype(float)
+        L, gfin = P.log_misfit(Nf, logT, mask, 1e-26)
+        dg, gN0 = hip.DeviceArray.from_host(gfin), hip.DeviceArray(len(N0))
+        gth = hip.DeviceArray(5)
+        h0 = hip.Event()
+        h0.record(s)
+        hip.check(adj.backward(dg, gN0, gth, s))
+        g2.record(s)
+        s.synchronize()
+        t_afwd, t_abwd = g0.elapsed_ms(g1), h0.elapsed_ms(g2)
+        ratio = (t_afwd + t_abwd) / t_fwd
+        out[name] = {"adjoint_forward_ms": t_afwd, "adjoint_backward_ms": t_abwd,
+                     "workspace_MB": adj.ws_bytes / 1e6, "gradient_over_forward": ratio}
+        if best is None or ratio < best[0]:
+            best = (ratio, name)
+    out["gradient_over_forward"], out["best_mode"] = best
+    return out
 
 
 def cpu_baseline(N, x, agg, brk, t_q, bw_q, budget_s):
@@ -144,6 +167,9 @@ def cpu_baseline(N, x, agg, brk, t_q, bw_q, budget_s):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(multi-GPU sharding is described in DESIGN.md)",

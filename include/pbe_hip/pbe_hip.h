@@ -228,11 +228,16 @@ hipError_t pbe_rhs_vjp(const double* N, const double* x, const double* t_q,
                        void* workspace, size_t workspace_bytes, hipStream_t stream);
 
 /* Workspace of the RK4 adjoint: ceil(steps/every) checkpoints plus the 4
- * stage states of one segment of `every` steps, plus O(n) scratch. */
+ * stage states of one segment of `every` steps, plus O(n) scratch.
+ * checkpoint_every = PBE_STORE_ALL_STAGES (0) keeps the 4 stage states of
+ * every step instead (4 steps n doubles): the reverse pass then recomputes
+ * nothing. */
+enum { PBE_STORE_ALL_STAGES = 0 };
 size_t pbe_adjoint_workspace_bytes(int n, int steps, int checkpoint_every);
 
 /* Forward pass: advance N in place by `steps` RK4 steps, storing a
- * checkpoint every `checkpoint_every` steps in the workspace. */
+ * checkpoint every `checkpoint_every` steps (or every stage state, for
+ * PBE_STORE_ALL_STAGES) in the workspace. */
 hipError_t pbe_adjoint_rk4_forward(double* N, const double* x, const double* t_q,
                                    const double* bw_q,
                                    const pbe_aggregation_params* agg,

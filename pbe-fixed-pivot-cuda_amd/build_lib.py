@@ -73,6 +73,15 @@ def build_cpp_tests():
     return out
 
 
+def build_host_checks():
+    """tests/cpp/test_roots.cpp: host-side check of the device cube root."""
+    src = os.path.join(REPO, "tests", "cpp", "test_roots.cpp")
+    out = os.path.join(REPO, "tests", "cpp", "test_roots")
+    _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", f"-I{CSRC}", src, "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_bench_helpers():
     """bench/batched_loop.cpp: batched launch vs a loop of plain calls."""
     src = os.path.join(HERE, "bench", "batched_loop.cpp")
@@ -86,6 +95,7 @@ def build_bench_helpers():
 def build(verbose=False):
     lib = build_library(verbose)
     exe = build_cpp_tests()
+    build_host_checks()
     build_bench_helpers()
     return lib, exe
 

@@ -41,10 +41,21 @@ __global__ __launch_bounds__(256)
 void agg_tile_kernel(AggArgs a)
 {
     __shared__ AggSmem<KT, U> sm;
+    constexpr int T = AggShape<U>::T, S = AggShape<U>::SPLIT, RW = AggShape<U>::RW;
+    const long long nb = a.nb;
+    const long long bt = band_tiles(nb);
+    const long long t = blockIdx.x;
     int p, d;
-    decode_tile(blockIdx.x, a.nb, p, d);
-    constexpr int T = AggShape<U>::T;
-    agg_tile_body<KT, U>(a, p * T, (p + d) * T, d == 0, sm);
+    if (t < S * bt) {
+        const long long tt = t / S;
+        const int q = (int)(t - tt * S);
+        d = tt < nb ? 0 : 1;
+        p = (int)(tt < nb ? tt : tt - nb);
+        agg_tile_body<KT, U>(a, p * T, (p + d) * T, d == 0, p * T + q * (T / S), RW / S, sm);
+    } else {
+        decode_tile(t - S * bt + bt, nb, p, d);
+        agg_tile_body<KT, U>(a, p * T, (p + d) * T, d == 0, p * T, RW, sm);
+    }
 }
 
 template <int KT, int U>
@@ -52,8 +63,9 @@ static hipError_t launch_agg_t(const AggArgs& args, hipStream_t stream)
 {
     const long long nb = args.nb;
     const long long tiles = nb * (nb + 1) / 2;
-    if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((agg_tile_kernel<KT, U>), dim3((unsigned)tiles), dim3(256),
+    const long long items = tiles + (AggShape<U>::SPLIT - 1) * band_tiles(nb);
+    if (items > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((agg_tile_kernel<KT, U>), dim3((unsigned)items), dim3(256),
                        0, stream, args);
     return hipSuccess;
 }

@@ -22,9 +22,19 @@ template <int U>
 struct AggShape {
     static constexpr int T = kWave * U;   // tile edge (rows == columns)
     static constexpr int RW = T / 4;      // rows per wave
+    static constexpr int SPLIT = 4;       // band tiles run as SPLIT row slices
     static constexpr int EXTRA = 256;     // near-birth reach beyond the tile
     static constexpr int WIN = T + EXTRA; // LDS window of target bins
 };
+
+// Work items of the single-instance launch: the 2nb - 1 tiles of the
+// diagonal band (d = 0, 1), whose near pairs cost several times a far pair,
+// first and split into SPLIT row slices each, then the far tiles whole, in
+// diagonal-major order.
+__host__ __device__ __forceinline__ long long band_tiles(long long nb)
+{
+    return nb >= 2 ? 2 * nb - 1 : nb;
+}
 
 // Tiles before diagonal d in diagonal-major order.
 __device__ __forceinline__ long long tiles_before(long long d, long long nb)
@@ -54,6 +64,8 @@ struct AggSmem {
     double xw[AggShape<U>::WIN + 1];    // pivots of the window bins
     double idw[AggShape<U>::WIN + 1];   // 1/(x_{i+1} - x_i) of the window bins
     double rowwin[AggShape<U>::T];      // row deaths of an off-diagonal tile
+    double colc[AggShape<U>::T];        // cube roots of the tile's column pivots
+    double colic[AggShape<U>::T];       // and their reciprocals
     double red[4][R];
 };
 
@@ -70,18 +82,21 @@ __device__ __forceinline__ void decode_tile(long long t, long long nb, int& p, i
     d = (int)dd;
 }
 
-// One pair tile: rows [j0, j0+T) x columns [k0, k0+T) of the upper triangle of
-// the instance described by `a`; `diag` marks the row block's diagonal tile,
-// which also adds that row block's death terms.  All 256 threads call it.
+// One work item of the pair tile with rows [j0, j0+T) and columns [k0, k0+T)
+// of the upper triangle of the instance described by `a`: the rows
+// [row_base, row_base + 4*rpw), rpw rows per wave (the whole tile, or one
+// slice of a costlier diagonal-band tile).  `diag` marks a diagonal tile.
+// All 256 threads call it.
 template <int KT, int U>
 __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, const int k0,
-                                              const bool diag, AggSmem<KT, U>& sm)
+                                              const bool diag, const int row_base,
+                                              const int rpw, AggSmem<KT, U>& sm)
 {
     using K = Kern<KT>;
     using S = AggShape<U>;
     constexpr int R = K::R;
     constexpr int RS = AggSmem<KT, U>::RS;
-    constexpr int RW = S::RW, WIN = S::WIN;
+    constexpr int WIN = S::WIN;
     auto& rowbuf = sm.rowbuf;
     double* colwin = sm.colwin;
     auto& red = sm.red;
@@ -98,15 +113,23 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
         sm.xw[i] = b < n ? a.x[b] : __builtin_inf();
         sm.idw[i] = b < n - 1 ? 1.0 / (a.x[b + 1] - a.x[b]) : 0.0;
     }
+    // cube roots of the column pivots, once per workgroup
+    for (int i = tid; i < AggShape<U>::T; i += 256) {
+        const int k = k0 + i;
+        double cb = 0.0, icb = 0.0;
+        if (k < n) pivot_roots<KT>(a.x[k], cb, icb);
+        sm.colc[i] = cb;
+        sm.colic[i] = icb;
+    }
 
     // ---- row factors of this wave's rows -> LDS ----
-    const int rw0 = j0 + wv * RW;
-    const int rw1 = min(rw0 + RW, n);
+    const int rw0 = row_base + wv * rpw;
+    const int rw1 = min(rw0 + rpw, n);
     double* myrows = rowbuf[wv];
     double tj[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) tj[r] = 0.0;
-    for (int rr = lane; rr < RW; rr += kWave) {
+    for (int rr = lane; rr < rpw; rr += kWave) {
         const int j = rw0 + rr;
         double av[R];
         double xj = 0.0;
@@ -132,26 +155,28 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
 #pragma unroll
     for (int r = 0; r < R; ++r) tj[r] = wave_sum(tj[r]);
 
-    // ---- column factors of this lane's columns ----
+    const double xlast = a.x[n - 1];
+    __syncthreads();
+
+    // ---- column factors of this lane's columns (from the shared tables) ----
     double Bc[U][R], xk[U], xk1[U], idx[U];
     int kc[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int k = k0 + lane + kWave * u;
+        const int c = lane + kWave * u;
+        const int k = k0 + c;
         kc[u] = k;
         if (k < n) {
-            const double xv = a.x[k];
+            const double xv = sm.xw[c];
             const double Nk = a.N[k];
-            double cb, icb, f[R];
-            pivot_roots<KT>(xv, cb, icb);
-            K::col(f, xv, cb, icb);
+            double f[R];
+            K::col(f, xv, sm.colc[c], sm.colic[c]);
 #pragma unroll
             for (int r = 0; r < R; ++r) Bc[u][r] = Nk * f[r];
             xk[u] = xv;
             if (k < n - 1) {
-                const double x1 = a.x[k + 1];
-                xk1[u] = x1;
-                idx[u] = 1.0 / (x1 - xv);
+                xk1[u] = sm.xw[c + 1];
+                idx[u] = sm.idw[c];
             } else {
                 xk1[u] = xv;     // never "far": every pair of column n-1 clips
                 idx[u] = 0.0;
@@ -164,8 +189,6 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
             idx[u] = 0.0;
         }
     }
-    const double xlast = a.x[n - 1];
-    __syncthreads();
 
     // ---- death, separable within the tile ----
     // Pairs (j < k) of this tile remove sum_k rate_jk from row j and
@@ -189,7 +212,7 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
         double TJ[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) TJ[r] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
-        for (int rr = lane; rr < RW; rr += kWave) {
+        for (int rr = lane; rr < rpw; rr += kWave) {
             const int j = rw0 + rr;
             if (j < n) {
                 double dth = 0.0;
@@ -272,70 +295,57 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
             auto idat = [&](int i) {
                 return (i - k0 <= WIN) ? sm.idw[i - k0] : 1.0 / (a.x[i + 1] - a.x[i]);
             };
-            int cur[U];
-            double xc[U], xc1[U], idc[U], accr[U], accw[U], accc[U];
+            // One column at a time (u outer) keeps a single column's cursor
+            // state live, so this path needs few more registers than the
+            // far path above.
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                cur[u] = kc[u];
-                xc[u] = xk[u];
-                xc1[u] = xk1[u];
-                idc[u] = idx[u];
-                accr[u] = 0.0;
-                accw[u] = 0.0;
-                accc[u] = 0.0;
-            }
-            for (int jj = 0; jj < nrows; ++jj) {
-                const int j = rw0 + jj;
-                const double* rp = myrows + jj * RS;
-                double av[R];
+                const int k = kc[u];
+                if (k >= n) continue;
+                int cur = k;
+                double xc = xk[u], xc1 = xk1[u], idc = idx[u];
+                double accr = 0.0, accw = 0.0, accc = 0.0;
+                const int jend = min(nrows, k - rw0 + 1);   // rows j <= k
+                for (int jj = 0; jj < jend; ++jj) {
+                    const int j = rw0 + jj;
+                    const double* rp = myrows + jj * RS;
+                    double pr = rp[0] * Bc[u][0];
 #pragma unroll
-                for (int r = 0; r < R; ++r) av[r] = rp[r];
-                const double xj = rp[R];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int k = kc[u];
-                    if (k < n && j <= k) {
-                        double pr = av[0] * Bc[u][0];
-#pragma unroll
-                        for (int r = 1; r < R; ++r) pr = fma(av[r], Bc[u][r], pr);
-                        if (j == k) pr *= 0.5;
-                        const double v = xj + xk[u];
-                        if (v >= xlast) {
-                            // upper clip (src/aggregation_kernels.cuh:191-192)
-                            accc[u] += pr;
-                        } else {
-                            if (v >= xc1[u]) {
-                                if (accr[u] != 0.0 || accw[u] != 0.0) {
-                                    sink.put(cur[u], accr[u] - accw[u]);
-                                    sink.put(cur[u] + 1, accw[u]);
-                                }
-                                int c = cur[u] + 1;
-                                if (xat(c + 1) <= v) {
-                                    const int hi = (a.inv_log_r != 0.0)
-                                        ? bracket_geometric(a.x, n, v, a.log_x0, a.inv_log_r)
-                                        : bracket_search(a.x, c, n - 1, v);
-                                    c = hi - 1;
-                                }
-                                cur[u] = c;
-                                xc[u] = xat(c);
-                                xc1[u] = xat(c + 1);
-                                idc[u] = idat(c);
-                                accr[u] = 0.0;
-                                accw[u] = 0.0;
+                    for (int r = 1; r < R; ++r) pr = fma(rp[r], Bc[u][r], pr);
+                    if (j == k) pr *= 0.5;
+                    const double v = rp[R] + xk[u];
+                    if (v >= xlast) {
+                        // upper clip (src/aggregation_kernels.cuh:191-192)
+                        accc += pr;
+                    } else {
+                        if (v >= xc1) {
+                            if (accr != 0.0 || accw != 0.0) {
+                                sink.put(cur, accr - accw);
+                                sink.put(cur + 1, accw);
                             }
-                            accr[u] += pr;
-                            accw[u] = fma((v - xc[u]) * idc[u], pr, accw[u]);
+                            int c = cur + 1;
+                            if (xat(c + 1) <= v) {
+                                const int hi = (a.inv_log_r != 0.0)
+                                    ? bracket_geometric(a.x, n, v, a.log_x0, a.inv_log_r)
+                                    : bracket_search(a.x, c, n - 1, v);
+                                c = hi - 1;
+                            }
+                            cur = c;
+                            xc = xat(c);
+                            xc1 = xat(c + 1);
+                            idc = idat(c);
+                            accr = 0.0;
+                            accw = 0.0;
                         }
+                        accr += pr;
+                        accw = fma((v - xc) * idc, pr, accw);
                     }
                 }
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (accr[u] != 0.0 || accw[u] != 0.0) {
-                    sink.put(cur[u], accr[u] - accw[u]);
-                    sink.put(cur[u] + 1, accw[u]);
+                if (accr != 0.0 || accw != 0.0) {
+                    sink.put(cur, accr - accw);
+                    sink.put(cur + 1, accw);
                 }
-                if (accc[u] != 0.0) sink.put(n - 1, accc[u]);
+                if (accc != 0.0) sink.put(n - 1, accc);
             }
         }
     }

@@ -53,7 +53,13 @@ void agg_batch_kernel(BatchArgs b)
         a.inv_log_r = d.inv_log_r;
         int p, dg;
         decode_tile(w.unit, d.nb, p, dg);
-        agg_tile_body<KT, kBatchU>(a, p * kBatchT, (p + dg) * kBatchT, dg == 0, sm);
+        constexpr int S = AggShape<kBatchU>::SPLIT, RW = AggShape<kBatchU>::RW;
+        if (w.slice >= 0)
+            agg_tile_body<KT, kBatchU>(a, p * kBatchT, (p + dg) * kBatchT, dg == 0,
+                                       p * kBatchT + w.slice * (kBatchT / S), RW / S, sm);
+        else
+            agg_tile_body<KT, kBatchU>(a, p * kBatchT, (p + dg) * kBatchT, dg == 0, p * kBatchT,
+                                       RW, sm);
         __syncthreads();
     }
 }
@@ -88,8 +94,8 @@ static double tile_cost(int n, int t)
     const long long j0 = p * kBatchT, k0 = (p + d) * kBatchT;
     const double rows = (double)std::min<long long>(kBatchT, n - j0);
     const double cols = (double)std::min<long long>(kBatchT, n - k0);
-    if (d == 0) return 2.0 * rows * (rows + 1) * 0.5 + 2.0 * n;  // mixed + death pass
-    if (d == 1) return 1.5 * rows * cols;                        // near band spill
+    if (d == 0) return 3.0 * rows * (rows + 1) * 0.5;   // near pairs cost several far ones
+    if (d == 1) return 2.0 * rows * cols;              // near band spill
     return rows * cols;
 }
 
@@ -133,8 +139,15 @@ hipError_t batch_plan_create(pbe_batch_plan* out, const pbe_batch_instance* hi, 
         for (int i = 0; i < count; ++i) {
             const long long nb = inst[i].nb;
             const long long tiles = nb * (nb + 1) / 2;
-            for (long long t = 0; t < tiles; ++t)
-                agg.push_back({tile_cost(inst[i].n, (int)t), BatchItem{i, (int)t}});
+            constexpr int S = AggShape<kBatchU>::SPLIT;
+            for (long long t = 0; t < tiles; ++t) {
+                const double c = tile_cost(inst[i].n, (int)t);
+                if (t < band_tiles(nb)) {
+                    for (int q = 0; q < S; ++q) agg.push_back({c / S, BatchItem{i, (int)t, q, 0}});
+                } else {
+                    agg.push_back({c, BatchItem{i, (int)t, -1, 0}});
+                }
+            }
         }
     }
     if (selection >= 0) {
@@ -142,7 +155,7 @@ hipError_t batch_plan_create(pbe_batch_plan* out, const pbe_batch_instance* hi, 
             const int blocks = (inst[i].n + kBrkParents - 1) / kBrkParents;
             for (int b = 0; b < blocks; ++b)
                 brk.push_back({(double)std::min(kBrkParents, inst[i].n - b * kBrkParents) * n_quad,
-                               BatchItem{i, b}});
+                               BatchItem{i, b, -1, 0}});
         }
     }
     std::vector<Costed> rows;
@@ -150,7 +163,7 @@ hipError_t batch_plan_create(pbe_batch_plan* out, const pbe_batch_instance* hi, 
         const int blocks = (inst[i].n + kWave - 1) / kWave;
         for (int b = 0; b < blocks; ++b)
             rows.push_back({(double)std::min(kWave, inst[i].n - b * kWave) * inst[i].n,
-                            BatchItem{i, b}});
+                            BatchItem{i, b, -1, 0}});
     }
     auto by_cost = [](const Costed& a, const Costed& b) {
         if (a.c != b.c) return a.c > b.c;

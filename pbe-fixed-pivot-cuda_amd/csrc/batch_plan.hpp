@@ -17,6 +17,8 @@ struct BatchInstDev {
 struct BatchItem {
     int inst;
     int unit;       // tile index (diagonal-major) or parent block index
+    int slice;      // row slice of a diagonal-band tile, -1 for a whole tile
+    int pad;
 };
 
 }  // namespace pbe

@@ -3,6 +3,8 @@
 // Check order and error codes follow src/aggregation.cu:23-24,54 and
 // src/breakage.cu:23-24,46,74 of the reference (n, then pointers, then the
 // enum).  Every check runs before the first HIP runtime call.
+#include <algorithm>
+
 #include "pbe_internal.hpp"
 
 static_assert(sizeof(pbe_aggregation_params) == 64, "AggregationParams layout");
@@ -304,7 +306,9 @@ extern "C" hipError_t pbe_graph_destroy(pbe_graph g)
 // --------------------------------------------------- VJP and RK4 adjoint ----
 extern "C" size_t pbe_vjp_workspace_bytes(int n)
 {
-    return n <= 0 ? 0 : (size_t)9 * (size_t)n * sizeof(double);
+    if (n <= 0) return 0;
+    const size_t part = std::max(pbe::vjp_part_doubles(n, false), pbe::vjp_part_doubles(n, true));
+    return ((size_t)9 * (size_t)n + part) * sizeof(double);
 }
 
 extern "C" hipError_t pbe_rhs_vjp(const double* N, const double* x, const double* t_q,
@@ -320,22 +324,23 @@ extern "C" hipError_t pbe_rhs_vjp(const double* N, const double* x, const double
     if (workspace_bytes < pbe_vjp_workspace_bytes(p.n)) return hipErrorInvalidValue;
     double* tab = static_cast<double*>(workspace);
     double* rows = tab + 4 * (size_t)p.n;
+    double* part = rows + 5 * (size_t)p.n;
     if ((e = pbe::enqueue_vjp_table(x, tab, p.n, stream)) != hipSuccess) return e;
     if ((e = pbe::enqueue_fill_zero(rows, 5 * p.n, stream)) != hipSuccess) return e;
-    if ((e = pbe::enqueue_vjp(p, N, lam, tab, gN, rows, stream)) != hipSuccess) return e;
+    if ((e = pbe::enqueue_vjp(p, N, lam, tab, gN, rows, part, stream)) != hipSuccess) return e;
     return pbe::enqueue_gtheta_reduce(rows, gtheta, p.n, stream);
 }
 
 extern "C" size_t pbe_adjoint_workspace_bytes(int n, int steps, int checkpoint_every)
 {
-    if (n <= 0 || steps <= 0 || checkpoint_every <= 0) return 0;
+    if (n <= 0 || steps <= 0 || checkpoint_every < 0) return 0;
     return pbe::adjoint_layout(n, steps, checkpoint_every).total * sizeof(double);
 }
 
 static hipError_t adjoint_checks(const pbe::RhsProblem& p, int steps, int every,
                                  const void* ws, size_t bytes)
 {
-    if (steps <= 0 || every <= 0) return hipErrorInvalidValue;
+    if (steps <= 0 || every < 0) return hipErrorInvalidValue;
     if (!ws) return hipErrorInvalidDevicePointer;
     if (bytes < pbe_adjoint_workspace_bytes(p.n, steps, every)) return hipErrorInvalidValue;
     return hipSuccess;
@@ -392,9 +397,10 @@ extern "C" hipError_t pbe_batch_rhs_vjp(pbe_batch_plan plan, const double* N, co
     if (workspace_bytes < pbe_vjp_workspace_bytes(p.n)) return hipErrorInvalidValue;
     double* tab = static_cast<double*>(workspace);
     double* rows = tab + 4 * (size_t)p.n;
+    double* part = rows + 5 * (size_t)p.n;
     if ((e = pbe::enqueue_vjp_table(x, tab, p.n, stream)) != hipSuccess) return e;
     if ((e = pbe::enqueue_fill_zero(rows, 5 * p.n, stream)) != hipSuccess) return e;
-    if ((e = pbe::enqueue_vjp(p, N, lam, tab, gN, rows, stream)) != hipSuccess) return e;
+    if ((e = pbe::enqueue_vjp(p, N, lam, tab, gN, rows, part, stream)) != hipSuccess) return e;
     return pbe::enqueue_gtheta_reduce(rows, gtheta, p.n, stream);
 }
 

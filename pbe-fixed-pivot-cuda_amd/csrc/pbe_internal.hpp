@@ -62,13 +62,25 @@ hipError_t enqueue_copy(const double* src, double* dst, int n, hipStream_t s);
 hipError_t enqueue_vjp_table(const double* x, double* tab, int n, hipStream_t s);
 // gN = lam . dR/dN at N (overwritten); gth_rows[n][5] += per-row parameter
 // partials (beta0, beta_br, beta_sh, S0, alpha); tab from enqueue_vjp_table.
+// part: scratch of vjp_part_doubles(n, batched) doubles.
+// fuse (optional) folds one RK4 adjoint stage into the VJP's last pass:
+//   lam_acc += gN;  av_next = c1 lam_base + c2 gN  (av_next may be null).
+struct AdjointStageFuse {
+    double* lam_acc;
+    double* av_next;
+    const double* lam_base;
+    double c1, c2;
+};
 hipError_t enqueue_vjp(const RhsProblem& p, const double* N, const double* lam,
-                       const double* tab, double* gN, double* gth_rows, hipStream_t s);
+                       const double* tab, double* gN, double* gth_rows, double* part,
+                       hipStream_t s, const AdjointStageFuse* fuse = nullptr);
+size_t vjp_part_doubles(int n, bool batched);
 
 // Workspace of the checkpointed RK4 adjoint, offsets in doubles.
 struct AdjointLayout {
+    bool store_all;
     int nseg;
-    size_t ckpt, seg, tab, lam, lam2, av, mu, rk, gth, total;
+    size_t ckpt, seg, tab, lam, lam2, av, mu, rk, gth, part, total;
 };
 AdjointLayout adjoint_layout(int n, int steps, int every);
 hipError_t enqueue_adjoint_forward(const RhsProblem& p, double* N, double dt, int steps,

@@ -41,6 +41,12 @@ struct VjpArgs {
     int sel;                          // -1: no breakage
     double S0, v_ref, alpha, v_min;
     int log_scale;                    // 1: slots 0-3 hold d/d ln(coefficient)
+    // Fused RK4 adjoint stage update (lam_acc null: plain VJP):
+    //   lam_acc += gN;  av_next = c1 lam_base + c2 gN (when av_next non-null)
+    double* lam_acc;
+    double* av_next;
+    const double* lam_base;
+    double c1, c2;
 };
 
 __global__ void vjp_table_kernel(const double* __restrict__ x, double* __restrict__ tab, int n)
@@ -103,19 +109,6 @@ template <> struct VjpGroups<5> {
     __device__ static int slot(int g) { return g == 0 ? 1 : 2; }
 };
 
-// eta(lam) of a near or clipped pair with volume v and larger index K.
-__device__ __forceinline__ double near_eta(const VjpArgs& a, double v, int K)
-{
-    const int n = a.n;
-    if (v >= a.x[n - 1]) return a.lam[n - 1];
-    if (v <= a.x[0]) return a.lam[0];
-    const int hi = bracket_search(a.x, K, n - 1, v);
-    const int lo = hi - 1;
-    const double xl = a.x[lo], xh = a.x[hi];
-    const double w = (v - xl) / (xh - xl);
-    return (1.0 - w) * a.lam[lo] + w * a.lam[hi];
-}
-
 __device__ __forceinline__ double selection_rt(int sel, double v, double S0, double v_ref,
                                                double alpha, double v_min)
 {
@@ -144,68 +137,72 @@ template <int KT> struct VjpSmem {
     double part[3][kWave][G];
 };
 
-// Rows [m0, m0 + 64) of the instance described by `a`; all 256 threads call
-// it.  KT < 0: no aggregation term.
 template <int KT>
-__device__ __forceinline__ void vjp_rows_body(const VjpArgs& a, const int m0, VjpSmem<KT>& sm)
+constexpr int vjp_groups() { return VjpSmem<KT>::G; }
+
+// Aggregation part of one work item: rows [m0, m0 + 64) against the columns
+// [kb, ke) of the instance described by `a`, each wave a quarter of them.
+// A lane keeps the exact bracket (cur, cur+1) of v = x_m + x_k, which only
+// moves up as k increases, so each pair costs a compare and, when the bracket
+// moves, a few loads (the same address for all lanes where every lane's pair
+// lands in (k, k+1)).  The four quarters are combined in a fixed order and
+// written to part[slot][m] (no atomics).
+template <int KT>
+__device__ __forceinline__ void vjp_part_body(const VjpArgs& a, const int m0, const int kb,
+                                              const int ke, double* __restrict__ out,
+                                              VjpSmem<KT>& sm)
 {
     constexpr int G = VjpSmem<KT>::G;
-    auto& part = sm.part;
-
+    using VG = VjpGroups<(KT >= 0 ? KT : 0)>;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int n = a.n;
     const int m = m0 + lane;
     const bool row = m < n;
-
     double acc[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) acc[g] = 0.0;
 
-    double Nm = 0.0, lm = 0.0, xm = 0.0;
-    if (row) {
-        Nm = a.N[m];
-        lm = a.lam[m];
-        xm = a.x[m];
-    }
-
-    if constexpr (KT >= 0) {
-        using VG = VjpGroups<KT>;
-        double xm1 = xm, im = 0.0, lm1 = lm, cm = 0.0, icm = 0.0;
-        if (row) {
-            cm = a.tab[4 * m + 0];
-            icm = a.tab[4 * m + 1];
-            im = a.tab[4 * m + 2];
-            if (m < n - 1) {
-                xm1 = a.x[m + 1];
-                lm1 = a.lam[m + 1];
+    const int q = (ke - kb + 3) / 4;
+    const int wb = kb + wv * q, we = min(ke, wb + q);
+    if (KT >= 0 && row && wb < we) {
+        const double xm = a.x[m], lm = a.lam[m];
+        const double cm = a.tab[4 * m + 0], icm = a.tab[4 * m + 1];
+        const double xlast = a.x[n - 1], llast = a.lam[n - 1];
+        // bracket of the first pair of this quarter
+        int cur = -1;
+        double xc = 0.0, xc1 = 0.0, lc = 0.0, lc1 = 0.0, idc = 0.0;
+        {
+            const double v0 = xm + a.x[wb];
+            if (v0 < xlast) {
+                cur = bracket_search(a.x, max(m, wb), n - 1, v0) - 1;
+                xc = a.x[cur];
+                xc1 = a.x[cur + 1];
+                lc = a.lam[cur];
+                lc1 = a.lam[cur + 1];
+                idc = a.tab[4 * cur + 2];
             }
         }
-        // this wave's quarter of the columns
-        const int chunk = (n + 3) / 4;
-        const int kb = wv * chunk, ke = min(n, kb + chunk);
-        for (int k = kb; k < ke; ++k) {
+        for (int k = wb; k < we; ++k) {
             const double xk = a.x[k];
             const double Nk = a.N[k];
             const double lk = a.lam[k];
             const double ck = a.tab[4 * k + 0];
             const double ick = a.tab[4 * k + 1];
-            const double ik = a.tab[4 * k + 2];
-            const double xk1 = (k < n - 1) ? a.x[k + 1] : xk;
-            const double lk1 = (k < n - 1) ? a.lam[k + 1] : lk;
-            if (!row) continue;
-            const bool kge = k >= m;
-            const double xK = kge ? xk : xm;
-            const double xK1 = kge ? xk1 : xm1;
-            const double iK = kge ? ik : im;
-            const double lK = kge ? lk : lm;
-            const double lK1 = kge ? lk1 : lm1;
             const double v = xm + xk;
             double eta;
-            if (v < xK1) {
-                const double w = (v - xK) * iK;
-                eta = fma(w, lK1 - lK, lK);
+            if (v >= xlast) {
+                eta = llast;                     // clipped to the last pivot
             } else {
-                eta = near_eta(a, v, kge ? k : m);
+                while (v >= xc1) {
+                    ++cur;
+                    xc = xc1;
+                    lc = lc1;
+                    xc1 = a.x[cur + 1];
+                    lc1 = a.lam[cur + 1];
+                    idc = a.tab[4 * cur + 2];
+                }
+                const double w = (v - xc) * idc;
+                eta = fma(w, lc1 - lc, lc);
             }
             const double t = Nk * ((eta - lm) - lk);
             double f[G];
@@ -214,21 +211,40 @@ __device__ __forceinline__ void vjp_rows_body(const VjpArgs& a, const int m0, Vj
             for (int g = 0; g < G; ++g) acc[g] = fma(f[g], t, acc[g]);
         }
     }
-
-    // combine the four column quarters in a fixed order
     if (wv > 0) {
 #pragma unroll
-        for (int g = 0; g < G; ++g) part[wv - 1][lane][g] = acc[g];
+        for (int g = 0; g < G; ++g) sm.part[wv - 1][lane][g] = acc[g];
     }
     __syncthreads();
-    if (wv != 0 || !row) return;
+    if (wv == 0 && row) {
 #pragma unroll
-    for (int g = 0; g < G; ++g) acc[g] = ((acc[g] + part[0][lane][g]) + part[1][lane][g]) + part[2][lane][g];
+        for (int g = 0; g < G; ++g)
+            out[(size_t)G * m + g] =
+                ((acc[g] + sm.part[0][lane][g]) + sm.part[1][lane][g]) + sm.part[2][lane][g];
+    }
+}
 
+// Per row: sum the column-chunk partials in order, form lam . dR/dN_m
+// (aggregation + breakage) and the row's parameter partials.
+template <int KT>
+__device__ __forceinline__ void vjp_finish_row(const VjpArgs& a, const int m,
+                                               const double* __restrict__ part, int nchunk,
+                                               size_t chunk_stride)
+{
+    constexpr int G = VjpSmem<KT>::G;
+    const int n = a.n;
+    const double Nm = a.N[m], lm = a.lam[m], xm = a.x[m];
     double gn = 0.0;
     double* gr = a.gth_rows + (size_t)kNP * m;
     if constexpr (KT >= 0) {
         using VG = VjpGroups<KT>;
+        double acc[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) acc[g] = 0.0;
+        for (int c = 0; c < nchunk; ++c) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) acc[g] += part[c * chunk_stride + (size_t)G * m + g];
+        }
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const double cf = VG::coef(g, a.coef);
@@ -261,43 +277,100 @@ __device__ __forceinline__ void vjp_rows_body(const VjpArgs& a, const int m0, Vj
         }
     }
     a.gN[m] = gn;
+    if (a.lam_acc) {
+        a.lam_acc[m] += gn;
+        if (a.av_next) a.av_next[m] = fma(a.c2, gn, a.c1 * a.lam_base[m]);
+    }
+}
+
+// Single instance.  Phase 1: grid (row blocks) x (column chunks).
+template <int KT>
+__global__ __launch_bounds__(256)
+void vjp_part_kernel(VjpArgs a, double* part, int kc)
+{
+    __shared__ VjpSmem<KT> sm;
+    constexpr int G = VjpSmem<KT>::G;
+    const int c = blockIdx.y;
+    const int kb = c * kc, ke = min(a.n, kb + kc);
+    vjp_part_body<KT>(a, blockIdx.x * kWave, kb, ke, part + (size_t)c * G * a.n, sm);
+}
+
+// Phase 2: one thread per row.
+template <int KT>
+__global__ __launch_bounds__(256)
+void vjp_finish_kernel(VjpArgs a, const double* part, int nchunk)
+{
+    constexpr int G = VjpSmem<KT>::G;
+    const int m = blockIdx.x * 256 + threadIdx.x;
+    if (m < a.n) vjp_finish_row<KT>(a, m, part, nchunk, (size_t)G * a.n);
+}
+
+// Batched.  Items are (instance, 64-row block); every instance is cut into
+// kBatchVjpChunks column chunks.  Slots 0-3 of the per-row partials hold
+// d/d ln(coefficient) so that their sum over instances is the gradient of
+// jointly scaled coefficients.
+constexpr int kBatchVjpChunks = 8;
+
+__device__ __forceinline__ VjpArgs instance_args(const VjpArgs& base, const BatchInstDev& d)
+{
+    VjpArgs a = base;
+    a.N = base.N + d.off;
+    a.x = base.x + d.off;
+    a.lam = base.lam + d.off;
+    a.tab = base.tab + 4 * (size_t)d.off;
+    a.gN = base.gN + d.off;
+    a.gth_rows = base.gth_rows + (size_t)kNP * d.off;
+    if (base.lam_acc) a.lam_acc = base.lam_acc + d.off;
+    if (base.av_next) {
+        a.av_next = base.av_next + d.off;
+        a.lam_base = base.lam_base + d.off;
+    }
+    a.n = d.n;
+    a.coef = KernelCoef{d.b0, d.b_br, d.b_sh};
+    a.S0 = d.S0;
+    a.v_ref = d.v_ref;
+    a.alpha = d.alpha;
+    a.v_min = d.v_min;
+    return a;
 }
 
 template <int KT>
 __global__ __launch_bounds__(256)
-void vjp_kernel(VjpArgs a)
+void vjp_batch_part_kernel(VjpArgs base, const BatchInstDev* __restrict__ inst,
+                           const BatchItem* __restrict__ items, int nitems, double* part,
+                           int total)
 {
     __shared__ VjpSmem<KT> sm;
-    vjp_rows_body<KT>(a, blockIdx.x * kWave, sm);
+    constexpr int G = VjpSmem<KT>::G;
+    const int nwork = nitems * kBatchVjpChunks;
+    for (int it = blockIdx.x; it < nwork; it += gridDim.x) {
+        const int c = it % kBatchVjpChunks;
+        const BatchItem w = items[it / kBatchVjpChunks];
+        const BatchInstDev d = inst[w.inst];
+        const VjpArgs a = instance_args(base, d);
+        const int kc = (d.n + kBatchVjpChunks - 1) / kBatchVjpChunks;
+        const int kb = min(d.n, c * kc), ke = min(d.n, kb + kc);
+        vjp_part_body<KT>(a, w.unit * kWave, kb, ke,
+                          part + ((size_t)c * total + d.off) * G, sm);
+        __syncthreads();
+    }
 }
 
-// Batched VJP: items are (instance, 64-row block); slots 0-3 of the per-row
-// partials hold d/d ln(coefficient) so that their sum over instances is the
-// gradient of jointly scaled coefficients.
 template <int KT>
 __global__ __launch_bounds__(256)
-void vjp_batch_kernel(VjpArgs base, const BatchInstDev* __restrict__ inst,
-                      const BatchItem* __restrict__ items, int nitems)
+void vjp_batch_finish_kernel(VjpArgs base, const BatchInstDev* __restrict__ inst,
+                             const BatchItem* __restrict__ items, int nitems,
+                             const double* part, int total)
 {
-    __shared__ VjpSmem<KT> sm;
+    constexpr int G = VjpSmem<KT>::G;
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const BatchItem w = items[it];
         const BatchInstDev d = inst[w.inst];
-        VjpArgs a = base;
-        a.N = base.N + d.off;
-        a.x = base.x + d.off;
-        a.lam = base.lam + d.off;
-        a.tab = base.tab + 4 * (size_t)d.off;
-        a.gN = base.gN + d.off;
-        a.gth_rows = base.gth_rows + (size_t)kNP * d.off;
-        a.n = d.n;
-        a.coef = KernelCoef{d.b0, d.b_br, d.b_sh};
-        a.S0 = d.S0;
-        a.v_ref = d.v_ref;
-        a.alpha = d.alpha;
-        a.v_min = d.v_min;
-        vjp_rows_body<KT>(a, w.unit * kWave, sm);
-        __syncthreads();
+        const VjpArgs a = instance_args(base, d);
+        const int m = w.unit * kWave + (int)threadIdx.x;
+        if (threadIdx.x < kWave && m < d.n)
+            vjp_finish_row<KT>(a, m, part + (size_t)G * d.off, kBatchVjpChunks,
+                               (size_t)G * total);
     }
 }
 
@@ -307,10 +380,62 @@ hipError_t enqueue_vjp_table(const double* x, double* tab, int n, hipStream_t s)
     return hipGetLastError();
 }
 
+// Column chunks of the single-instance VJP for n bins: enough work items to
+// fill the device (about 1024), at least 16 columns per chunk.
+static int vjp_chunks(int n)
+{
+    const int nrb = (n + kWave - 1) / kWave;
+    int c = (1024 + nrb - 1) / nrb;
+    c = std::min(c, 64);
+    c = std::min(c, std::max(1, n / 16));
+    return std::max(c, 1);
+}
+
+size_t vjp_part_doubles(int n, bool batched)
+{
+    return batched ? (size_t)kBatchVjpChunks * 2 * (size_t)n
+                   : (size_t)vjp_chunks(n) * 2 * (size_t)n;
+}
+
+template <int KT>
+static void launch_vjp_single(const VjpArgs& a, double* part, hipStream_t s)
+{
+    const int nch = vjp_chunks(a.n);
+    const int kc = (a.n + nch - 1) / nch;
+    const dim3 g1((unsigned)((a.n + kWave - 1) / kWave), (unsigned)nch), b(256);
+    if (KT >= 0) hipLaunchKernelGGL(vjp_part_kernel<KT>, g1, b, 0, s, a, part, kc);
+    hipLaunchKernelGGL(vjp_finish_kernel<KT>, dim3((unsigned)((a.n + 255) / 256)), b, 0, s, a,
+                       (const double*)part, nch);
+}
+
+template <int KT>
+static void launch_vjp_batch(const VjpArgs& a, const pbe_batch_plan_s* pl, double* part,
+                             hipStream_t s)
+{
+    const dim3 b(256);
+    const int nw = pl->n_row_items * kBatchVjpChunks;
+    if (KT >= 0)
+    hipLaunchKernelGGL(vjp_batch_part_kernel<KT>, dim3((unsigned)std::min(pl->grid, nw)), b, 0,
+                       s, a, (const BatchInstDev*)pl->d_inst, (const BatchItem*)pl->d_row_items,
+                       pl->n_row_items, part, pl->total);
+    hipLaunchKernelGGL(vjp_batch_finish_kernel<KT>,
+                       dim3((unsigned)std::min(pl->grid, pl->n_row_items)), b, 0, s, a,
+                       (const BatchInstDev*)pl->d_inst, (const BatchItem*)pl->d_row_items,
+                       pl->n_row_items, (const double*)part, pl->total);
+}
+
 hipError_t enqueue_vjp(const RhsProblem& p, const double* N, const double* lam,
-                       const double* tab, double* gN, double* gth_rows, hipStream_t s)
+                       const double* tab, double* gN, double* gth_rows, double* part,
+                       hipStream_t s, const AdjointStageFuse* fuse)
 {
     VjpArgs a{};
+    if (fuse) {
+        a.lam_acc = fuse->lam_acc;
+        a.av_next = fuse->av_next;
+        a.lam_base = fuse->lam_base;
+        a.c1 = fuse->c1;
+        a.c2 = fuse->c2;
+    }
     a.N = N;
     a.x = p.x;
     a.t_q = p.t_q;
@@ -320,6 +445,25 @@ hipError_t enqueue_vjp(const RhsProblem& p, const double* N, const double* lam,
     a.gN = gN;
     a.gth_rows = gth_rows;
     a.n = p.n;
+    int kt = -1;
+    if (p.batch) {
+        const pbe_batch_plan_s* pl = p.batch;
+        a.nq = pl->selection >= 0 ? pl->n_quad : 0;
+        a.sel = pl->selection;
+        a.log_scale = 1;
+        kt = pl->kernel_type;
+        switch (kt) {
+        case -1: launch_vjp_batch<-1>(a, pl, part, s); break;
+        case 0: launch_vjp_batch<0>(a, pl, part, s); break;
+        case 1: launch_vjp_batch<1>(a, pl, part, s); break;
+        case 2: launch_vjp_batch<2>(a, pl, part, s); break;
+        case 3: launch_vjp_batch<3>(a, pl, part, s); break;
+        case 4: launch_vjp_batch<4>(a, pl, part, s); break;
+        case 5: launch_vjp_batch<5>(a, pl, part, s); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     a.nq = p.brk ? p.brk->n_quad : 0;
     a.sel = p.brk ? p.brk->selection : -1;
     if (p.agg) a.coef = KernelCoef{p.agg->beta0, p.agg->beta_br, p.agg->beta_sh};
@@ -329,36 +473,15 @@ hipError_t enqueue_vjp(const RhsProblem& p, const double* N, const double* lam,
         a.alpha = p.brk->alpha;
         a.v_min = p.brk->v_min;
     }
-    if (p.batch) {
-        const pbe_batch_plan_s* pl = p.batch;
-        a.nq = pl->selection >= 0 ? pl->n_quad : 0;
-        a.sel = pl->selection;
-        a.log_scale = 1;
-        const dim3 g((unsigned)std::min(pl->grid, pl->n_row_items)), b(256);
-        const BatchInstDev* di = pl->d_inst;
-        const BatchItem* dr = pl->d_row_items;
-        const int ni = pl->n_row_items;
-        switch (pl->kernel_type) {
-        case -1: hipLaunchKernelGGL(vjp_batch_kernel<-1>, g, b, 0, s, a, di, dr, ni); break;
-        case 0: hipLaunchKernelGGL(vjp_batch_kernel<0>, g, b, 0, s, a, di, dr, ni); break;
-        case 1: hipLaunchKernelGGL(vjp_batch_kernel<1>, g, b, 0, s, a, di, dr, ni); break;
-        case 2: hipLaunchKernelGGL(vjp_batch_kernel<2>, g, b, 0, s, a, di, dr, ni); break;
-        case 3: hipLaunchKernelGGL(vjp_batch_kernel<3>, g, b, 0, s, a, di, dr, ni); break;
-        case 4: hipLaunchKernelGGL(vjp_batch_kernel<4>, g, b, 0, s, a, di, dr, ni); break;
-        case 5: hipLaunchKernelGGL(vjp_batch_kernel<5>, g, b, 0, s, a, di, dr, ni); break;
-        default: return hipErrorInvalidValue;
-        }
-        return hipGetLastError();
-    }
-    const dim3 g((unsigned)((p.n + kWave - 1) / kWave)), b(256);
-    switch (p.agg ? p.agg->kernel_type : -1) {
-    case -1: hipLaunchKernelGGL(vjp_kernel<-1>, g, b, 0, s, a); break;
-    case 0: hipLaunchKernelGGL(vjp_kernel<0>, g, b, 0, s, a); break;
-    case 1: hipLaunchKernelGGL(vjp_kernel<1>, g, b, 0, s, a); break;
-    case 2: hipLaunchKernelGGL(vjp_kernel<2>, g, b, 0, s, a); break;
-    case 3: hipLaunchKernelGGL(vjp_kernel<3>, g, b, 0, s, a); break;
-    case 4: hipLaunchKernelGGL(vjp_kernel<4>, g, b, 0, s, a); break;
-    case 5: hipLaunchKernelGGL(vjp_kernel<5>, g, b, 0, s, a); break;
+    kt = p.agg ? p.agg->kernel_type : -1;
+    switch (kt) {
+    case -1: launch_vjp_single<-1>(a, part, s); break;
+    case 0: launch_vjp_single<0>(a, part, s); break;
+    case 1: launch_vjp_single<1>(a, part, s); break;
+    case 2: launch_vjp_single<2>(a, part, s); break;
+    case 3: launch_vjp_single<3>(a, part, s); break;
+    case 4: launch_vjp_single<4>(a, part, s); break;
+    case 5: launch_vjp_single<5>(a, part, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -379,18 +502,6 @@ __global__ void adj_start_kernel(const double* __restrict__ lam, double* __restr
     const double l = lam[i];
     lam_new[i] = l;
     av[i] = c * l;
-}
-
-// lam_new += mu; av = c1 lam + c2 mu (next stage's cotangent)
-__global__ void adj_stage_kernel(const double* __restrict__ lam, const double* __restrict__ mu,
-                                 double* __restrict__ lam_new, double* __restrict__ av, int n,
-                                 double c1, double c2, int last)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double m = mu[i];
-    lam_new[i] += m;
-    if (!last) av[i] = fma(c2, m, c1 * lam[i]);
 }
 
 __global__ void copy_kernel(const double* __restrict__ src, double* __restrict__ dst, int n)
@@ -432,21 +543,28 @@ hipError_t enqueue_copy(const double* src, double* dst, int n, hipStream_t s)
     return hipGetLastError();
 }
 
+// every > 0: a checkpoint every `every` steps and the stage states of one
+// segment, recomputed in the reverse pass.  every == 0: the forward pass keeps
+// the four stage states of every step (4 steps n doubles) and the reverse
+// pass recomputes nothing.
 AdjointLayout adjoint_layout(int n, int steps, int every)
 {
     AdjointLayout L{};
     const size_t nn = (size_t)n;
-    L.nseg = (steps + every - 1) / every;
+    L.store_all = every == 0;
+    L.nseg = L.store_all ? 1 : (steps + every - 1) / every;
+    const size_t seg_steps = L.store_all ? (size_t)steps : (size_t)every;
     size_t off = 0;
-    L.ckpt = off;  off += nn * (size_t)L.nseg;
-    L.seg = off;   off += nn * 4 * (size_t)every;
+    L.ckpt = off;  off += L.store_all ? 0 : nn * (size_t)L.nseg;
+    L.seg = off;   off += nn * 4 * seg_steps;
     L.tab = off;   off += 4 * nn;
     L.lam = off;   off += nn;
     L.lam2 = off;  off += nn;
-    L.av = off;    off += nn;
+    L.av = off;    off += 2 * nn;
     L.mu = off;    off += nn;
     L.rk = off;    off += 3 * nn;
     L.gth = off;   off += (size_t)kNP * nn;
+    L.part = off;  off += std::max(vjp_part_doubles(n, false), vjp_part_doubles(n, true));
     L.total = off; // doubles
     return L;
 }
@@ -456,9 +574,17 @@ hipError_t enqueue_adjoint_forward(const RhsProblem& p, double* N, double dt, in
 {
     const AdjointLayout L = adjoint_layout(p.n, steps, every);
     double* rk = ws + L.rk;
+    const size_t nn = (size_t)p.n;
     hipError_t e = enqueue_fill_zero(rk, p.n, s);
     for (int i = 0; e == hipSuccess && i < steps; ++i) {
-        if (i % every == 0) e = enqueue_copy(N, ws + L.ckpt + (size_t)p.n * (i / every), p.n, s);
+        if (L.store_all) {
+            double* Yi = ws + L.seg + nn * 4 * (size_t)i;
+            double* st[3] = {Yi + nn, Yi + 2 * nn, Yi + 3 * nn};
+            e = enqueue_copy(N, Yi, p.n, s);
+            if (e == hipSuccess) e = enqueue_rk4_step_stages(p, N, dt, rk, st, s);
+            continue;
+        }
+        if (i % every == 0) e = enqueue_copy(N, ws + L.ckpt + nn * (i / every), p.n, s);
         if (e == hipSuccess) e = enqueue_rk4_step_stages(p, N, dt, rk, nullptr, s);
     }
     return e;
@@ -474,11 +600,13 @@ hipError_t enqueue_adjoint_backward(const RhsProblem& p, const double* gN_final,
     double* tab = ws + L.tab;
     double* lam = ws + L.lam;
     double* lam2 = ws + L.lam2;
-    double* av = ws + L.av;
+    double* av0 = ws + L.av;
+    double* av1 = ws + L.av + nn;
     double* mu = ws + L.mu;
     double* rk = ws + L.rk;
     double* gth = ws + L.gth;
     double* seg = ws + L.seg;
+    double* part = ws + L.part;
     hipError_t e;
     if ((e = enqueue_vjp_table(p.x, tab, n, s)) != hipSuccess) return e;
     if ((e = enqueue_fill_zero(gth, kNP * n, s)) != hipSuccess) return e;
@@ -486,11 +614,13 @@ hipError_t enqueue_adjoint_backward(const RhsProblem& p, const double* gN_final,
     if ((e = enqueue_copy(gN_final, lam, n, s)) != hipSuccess) return e;
     const dim3 g = grid1(n), b(256);
     for (int sg = L.nseg - 1; sg >= 0; --sg) {
-        const int s0 = sg * every, s1 = min(steps, s0 + every);
-        // recompute the segment, keeping every stage state
+        const int s0 = L.store_all ? 0 : sg * every;
+        const int s1 = L.store_all ? steps : min(steps, s0 + every);
         double* Y = seg;   // Y[(i - s0)*4 + q]
-        if ((e = enqueue_copy(ws + L.ckpt + nn * sg, Y, n, s)) != hipSuccess) return e;
-        for (int i = s0; i < s1; ++i) {
+        // recompute the segment, keeping every stage state
+        if (!L.store_all && (e = enqueue_copy(ws + L.ckpt + nn * sg, Y, n, s)) != hipSuccess)
+            return e;
+        for (int i = s0; i < s1 && !L.store_all; ++i) {
             double* Yi = Y + nn * 4 * (size_t)(i - s0);
             double* st[3] = {Yi + nn, Yi + 2 * nn, Yi + 3 * nn};
             // the step advances a copy: Y1 of step i+1 follows Y4 of step i
@@ -504,18 +634,22 @@ hipError_t enqueue_adjoint_backward(const RhsProblem& p, const double* gN_final,
                 if ((e = enqueue_rk4_step_stages(p, mu, dt, rk, st, s)) != hipSuccess) return e;
             }
         }
-        // reverse sweep over the segment
+        // reverse sweep over the segment; each VJP's finish pass also adds
+        // mu_s to the new lam and forms the next stage's cotangent
         for (int i = s1 - 1; i >= s0; --i) {
             double* Yi = Y + nn * 4 * (size_t)(i - s0);
-            hipLaunchKernelGGL(adj_start_kernel, g, b, 0, s, lam, lam2, av, n, dt / 6.0);
-            if ((e = enqueue_vjp(p, Yi + 3 * nn, av, tab, mu, gth, s)) != hipSuccess) return e;
-            hipLaunchKernelGGL(adj_stage_kernel, g, b, 0, s, lam, mu, lam2, av, n, dt / 3.0, dt, 0);
-            if ((e = enqueue_vjp(p, Yi + 2 * nn, av, tab, mu, gth, s)) != hipSuccess) return e;
-            hipLaunchKernelGGL(adj_stage_kernel, g, b, 0, s, lam, mu, lam2, av, n, dt / 3.0, 0.5 * dt, 0);
-            if ((e = enqueue_vjp(p, Yi + nn, av, tab, mu, gth, s)) != hipSuccess) return e;
-            hipLaunchKernelGGL(adj_stage_kernel, g, b, 0, s, lam, mu, lam2, av, n, dt / 6.0, 0.5 * dt, 0);
-            if ((e = enqueue_vjp(p, Yi, av, tab, mu, gth, s)) != hipSuccess) return e;
-            hipLaunchKernelGGL(adj_stage_kernel, g, b, 0, s, lam, mu, lam2, av, n, 0.0, 0.0, 1);
+            hipLaunchKernelGGL(adj_start_kernel, g, b, 0, s, lam, lam2, av0, n, dt / 6.0);
+            const AdjointStageFuse f4{lam2, av1, lam, dt / 3.0, dt};
+            if ((e = enqueue_vjp(p, Yi + 3 * nn, av0, tab, mu, gth, part, s, &f4)) != hipSuccess)
+                return e;
+            const AdjointStageFuse f3{lam2, av0, lam, dt / 3.0, 0.5 * dt};
+            if ((e = enqueue_vjp(p, Yi + 2 * nn, av1, tab, mu, gth, part, s, &f3)) != hipSuccess)
+                return e;
+            const AdjointStageFuse f2{lam2, av1, lam, dt / 6.0, 0.5 * dt};
+            if ((e = enqueue_vjp(p, Yi + nn, av0, tab, mu, gth, part, s, &f2)) != hipSuccess)
+                return e;
+            const AdjointStageFuse f1{lam2, nullptr, nullptr, 0.0, 0.0};
+            if ((e = enqueue_vjp(p, Yi, av1, tab, mu, gth, part, s, &f1)) != hipSuccess) return e;
             double* t = lam;
             lam = lam2;
             lam2 = t;

@@ -293,6 +293,7 @@ class BatchPlan:
 
 # ---------------------------------------------------------- differentiation --
 GRAD_SLOTS = ("beta0", "beta_br", "beta_sh", "S0", "alpha")
+STORE_ALL_STAGES = 0   # checkpoint interval that keeps every RK4 stage state
 
 
 def rhs_vjp(N, x, lam, agg=None, brk=None, t_q=None, bw_q=None):
@@ -315,7 +316,8 @@ def rhs_vjp(N, x, lam, agg=None, brk=None, t_q=None, bw_q=None):
 
 class Adjoint:
     """Checkpointed discrete RK4 adjoint (pbe_adjoint_rk4_forward/backward)
-    of `steps` steps of size dt, checkpoint every `every` steps."""
+    of `steps` steps of size dt, checkpoint every `every` steps (every = 0:
+    keep every stage state, no recompute in the reverse pass)."""
 
     def __init__(self, x, dt, steps, every=50, agg=None, brk=None, t_q=None, bw_q=None):
         self.it = Integrator(x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)

@@ -117,3 +117,17 @@ def test_daughter_tables():
             assert abs((bw * t).sum() - 1.0) < 1e-13
     with pytest.raises(ValueError):
         P.daughter_power_law(2.0, 7)
+
+
+def test_device_cube_root_host_check():
+    """cbrt_pair (the bit-guess + Newton cube root of the kernels), compiled
+    for the host from the same header: within 3 ulp of the C library's cbrt,
+    its reciprocal within 6 ulp (tests/cpp/test_roots.cpp)."""
+    import json
+    import subprocess
+    exe = os.path.join(REPO, "tests", "cpp", "test_roots")
+    assert os.path.exists(exe), "tests/cpp/test_roots not built (run build())"
+    r = subprocess.run([exe], stdout=subprocess.PIPE, text=True, timeout=120)
+    out = json.loads(r.stdout)
+    assert r.returncode == 0, out
+    assert out["cbrt_max_ulp"] <= 4 and out["inv_cbrt_max_ulp"] <= 8

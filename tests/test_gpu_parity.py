@@ -167,7 +167,9 @@ def test_accumulates_into_rhs(pbe, oracle_mod):
     hip.synchronize()
     got = drhs.download()
     ref, A, _ = _oracle(oracle_mod, N, x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
-    assert_parity(got - base, ref, A + np.abs(base) * 1e-4, what="accumulate")
+    # each atomic add into a bin holding ~|base| rounds at ulp(|base|)
+    slack = 64 * np.spacing(np.abs(base) + np.abs(ref))
+    assert np.all(np.abs(got - base - ref) <= 1e-12 * A + slack)
 
 
 def test_flocculation_16384_vs_port(pbe, oracle_mod):
