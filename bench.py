@@ -139,6 +139,36 @@ def adjoint_form(pbe, hip, P):
     return out
 
 
+def rhs_variant(pbe_hip, hip, P, n, daughter, steps=20):
+    """Average step time (zero rhs, aggregation, breakage) of one flocculation
+    RHS configuration, HIP events on the launch stream."""
+    x, N, agg, brk, t_q, bw_q = P.config_flocculation(n, daughter=daughter, n_quad=16)
+    nq = len(t_q)
+    dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
+    dt, db = hip.DeviceArray.from_host(t_q), hip.DeviceArray.from_host(bw_q)
+    drhs = hip.DeviceArray(n)
+    ap, bp = pbe_hip.agg_params(n, agg), pbe_hip.brk_params(n, nq, brk)
+    s = hip.Stream()
+    rt = hip.runtime()
+
+    def step():
+        hip.check(rt.hipMemsetAsync(drhs.ptr, 0, drhs.nbytes, s.handle), "memset")
+        hip.check(pbe_hip.launch_aggregation_rhs(dN, dx, drhs, ap, s), "agg")
+        hip.check(pbe_hip.launch_breakage_rhs(dN, dx, dt, db, drhs, bp, s), "brk")
+
+    for _ in range(3):
+        step()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(s)
+    for _ in range(steps):
+        step()
+    e1.record(s)
+    s.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    return {"n": n, "daughter": daughter, "n_quad": nq, "ms_per_step": ms,
+            "pair_updates_per_s": P.pair_units(n, nq) / (ms * 1e-3)}
+
+
 def cpu_baseline(N, x, agg, brk, t_q, bw_q, budget_s):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
@@ -280,6 +310,8 @@ def main():
         "ms_per_step_without_memset": ms_step_nomemset,
         "wall_s_timed_region": wall,
     }
+    out["variants"] = [rhs_variant(pbe_hip, hip, P, m, "symmetric") for m in (1024, 4096)]
+    out["variants"].append(rhs_variant(pbe_hip, hip, P, n, "uniform"))
     if not args.no_forms:
         out["forms"] = {"batched": batched_form(hip, P), "adjoint": adjoint_form(pbe_hip, hip, P)}
     if not args.no_cpu:

@@ -88,12 +88,15 @@ hipError_t pbe_launch_aggregation_rhs(const double* N, const double* x,
                                       const pbe_aggregation_params* params,
                                       hipStream_t stream);
 
-/* Launch options of the aggregation tiles (no counterpart in the reference,
+/* Launch options of the aggregation kernel (no counterpart in the reference,
  * whose only knob is the compile-time PBE_TILE, src/aggregation_kernels.cuh:
- * 30-33).  tile_u = columns per lane: the pair tiles are 64*tile_u square;
- * 0 picks the library's autotuned choice for (n, kernel_type). */
+ * 30-33).  tile_u = columns per lane: column strips are 64*tile_u wide;
+ * near_weight = the cost of a 64-row chunk of near-diagonal pairs relative
+ * to a chunk of far pairs, used to split the work evenly over the resident
+ * workgroups.  0 picks the library's autotuned choice for (n, kernel_type). */
 typedef struct pbe_agg_options {
-    int tile_u;   /* 0, 1, 2, 4 or 8 */
+    int tile_u;        /* 0, 1, 2, 4 or 8 */
+    int near_weight;   /* 0 or 1..16 */
 } pbe_agg_options;
 
 /* pbe_launch_aggregation_rhs with explicit launch options (same results). */
@@ -120,6 +123,32 @@ hipError_t pbe_launch_breakage_rhs(const double* N, const double* x,
                                    double* rhs,
                                    const pbe_breakage_params* params,
                                    hipStream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Atomic-free gather mode (no counterpart in the reference, whose FP64      */
+/* atomics make the summation order, and so the last bits, vary from run to */
+/* run).  Same contract and errors as the launches above, plus a caller-    */
+/* owned device workspace of at least the returned size                     */
+/* (hipErrorInvalidValue if smaller, hipErrorInvalidDevicePointer if NULL). */
+/* Every rhs entry is written by one thread and every sum is formed in an   */
+/* order fixed by n (and n_quad) alone: two calls on the same inputs give   */
+/* identical bits.  Results agree with the atomic launches to rounding.     */
+/* ------------------------------------------------------------------------ */
+size_t pbe_aggregation_deterministic_workspace_bytes(int n);
+size_t pbe_breakage_deterministic_workspace_bytes(int n);
+
+hipError_t pbe_launch_aggregation_rhs_deterministic(const double* N, const double* x,
+                                                    double* rhs,
+                                                    const pbe_aggregation_params* params,
+                                                    void* workspace, size_t workspace_bytes,
+                                                    hipStream_t stream);
+
+hipError_t pbe_launch_breakage_rhs_deterministic(const double* N, const double* x,
+                                                 const double* t_q, const double* bw_q,
+                                                 double* rhs,
+                                                 const pbe_breakage_params* params,
+                                                 void* workspace, size_t workspace_bytes,
+                                                 hipStream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* Device time integration (no counterpart in the reference library: it     */

@@ -1,14 +1,15 @@
-"""Pick the aggregation pair-tile shape per (n, kernel) by measurement.
+"""Pick the aggregation strip shape per (n, kernel) by measurement.
 
     python pbe-fixed-pivot-cuda_amd/autotune.py --out tile_table.inc [--json results.json]
 
-For each size bucket and collision kernel, every tile shape (64*U square
-tiles, U = 1, 2, 4, 8 columns per lane) is timed with HIP events over
-back-to-back launches on the flocculation workload (lognormal N, geometric
-grid r = 1.02), and the fastest U is written as a rule {n_max, kernel, U} in
-the C include that the library compiles in (csrc/tile_table.inc).  The table
-is read-only data built offline: the library holds no tuning state at run
-time.
+For each size bucket and collision kernel, every strip width (64*U columns,
+U = 1, 2, 4, 8 columns per lane) and near-chunk weight W (the cost of a chunk
+of near-diagonal pairs relative to a far chunk, used to split the work over
+the resident workgroups) is timed with HIP events over back-to-back launches
+on the flocculation workload (lognormal N, geometric grid r = 1.02), and the
+fastest (U, W) is written as a rule {n_max, kernel, U, W} in the C include
+that the library compiles in (csrc/tile_table.inc).  The table is read-only
+data built offline: the library holds no tuning state at run time.
 """
 import argparse
 import json
@@ -21,23 +22,20 @@ sys.path.insert(0, os.path.join(HERE, "python"))
 SIZES = [256, 512, 1024, 2048, 4096, 8192, 16384]
 KERNELS = [0, 1, 2, 3, 4, 5]
 TILES = [1, 2, 4, 8]
+WEIGHTS = [1, 2, 3, 4, 6]
 
 
-def time_launch(pbe, hip, N, x, agg, u, reps):
-    import ctypes
+def time_launch(pbe, hip, N, x, agg, u, w, reps):
     n = len(N)
     dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
     drhs = hip.DeviceArray(n)
     drhs.zero()
     p = pbe.agg_params(n, agg)
-    opt = (ctypes.c_int * 1)(u)
     s = hip.Stream()
-    L = pbe.library()
 
     def launch():
-        hip.check(L.pbe_launch_aggregation_rhs_ex(dN.ptr, dx.ptr, drhs.ptr, ctypes.byref(p),
-                                                  ctypes.cast(opt, ctypes.c_void_p),
-                                                  s.handle), f"tile {u}")
+        hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=u, near_weight=w,
+                                                stream=s), f"tile {u} weight {w}")
     for _ in range(3):
         launch()
     s.synchronize()
@@ -66,23 +64,27 @@ def main():
         for k in KERNELS:
             agg = dict(agg0, kernel=k, beta0=1e-17 / (x[n // 4] ** (2 if k == 2 else 1 if k == 1 else 0)))
             reps = max(5, min(200, int(4e9 / (n * n))))
-            t = {u: time_launch(pbe, hip, N, x, agg, u, reps) for u in TILES}
-            results[f"{n}/{k}"] = t
+            t = {(u, w): time_launch(pbe, hip, N, x, agg, u, w, reps)
+                 for u in TILES for w in WEIGHTS}
+            results[f"{n}/{k}"] = {f"U{u}W{w}": v for (u, w), v in t.items()}
             best[(n, k)] = min(t, key=t.get)
-            print(f"n={n:6d} kernel={k} " + " ".join(f"U{u}:{t[u] * 1e3:8.1f}us" for u in TILES)
-                  + f"  -> U={best[(n, k)]}", flush=True)
-    lines = ["// {n_max, kernel_type (-1 = any), columns per lane}; first match wins.",
-             "// Generated by pbe-fixed-pivot-cuda_amd/autotune.py on an MI355X (gfx950)."]
+            bu, bw = best[(n, k)]
+            print(f"n={n:6d} kernel={k} " + " ".join(
+                f"U{u}:{min(t[(u, w)] for w in WEIGHTS) * 1e3:8.1f}us" for u in TILES)
+                + f"  -> U={bu} W={bw} {t[(bu, bw)] * 1e3:.1f}us", flush=True)
+    lines = ["// {n_max, kernel_type (-1 = any), columns per lane, near-chunk weight};",
+             "// first match wins.  Generated by pbe-fixed-pivot-cuda_amd/autotune.py on",
+             "// an MI355X (gfx950)."]
     for k in KERNELS:
         prev = None
         for i, n in enumerate(sizes):
-            u = best[(n, k)]
+            u, w = best[(n, k)]
             nmax = (sizes[i] + sizes[i + 1]) // 2 if i + 1 < len(sizes) else (1 << 30)
-            if prev is not None and prev[1] == u:
-                lines[-1] = f"{{{nmax}, {k}, {u}}},"
+            if prev is not None and prev[1] == (u, w):
+                lines[-1] = f"{{{nmax}, {k}, {u}, {w}}},"
             else:
-                lines.append(f"{{{nmax}, {k}, {u}}},")
-            prev = (nmax, u)
+                lines.append(f"{{{nmax}, {k}, {u}, {w}}},")
+            prev = (nmax, (u, w))
     with open(args.out, "w") as f:
         f.write("\n".join(lines) + "\n")
     if args.json:

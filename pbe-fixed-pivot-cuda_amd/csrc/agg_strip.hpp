@@ -1,0 +1,387 @@
+// agg_strip.hpp -- single-instance aggregation as column strips split evenly
+// over a resident grid (device code; the launcher is in agg_kernels.hip).
+//
+// Work decomposition.  Column block q holds the T = 64*U columns
+// [qT, qT + T); its pairs are the rows j <= k, cut into chunks of 64 rows.
+// The chunks are laid out block after block, rows ascending within a block,
+// and weighted by an estimate of their cost: 1 for a chunk whose rows all lie
+// at least one block below the block (far pairs), W for the two blocks of
+// rows at and just below the diagonal (near pairs, cursor path).  The grid is
+// exactly the number of workgroups the device holds at once, and workgroup g
+// takes the chunks whose cumulative cost starts in [g C / G, (g+1) C / G):
+// every workgroup is resident from the start and all finish together, and
+// a workgroup usually stays within one or two column blocks, so the column
+// set-up (pivots, cube roots, the LDS window) and the flush of the window are
+// paid once per visit of a block rather than once per 64 x T tile.
+//
+// Within a visit the four waves take the visit's chunks round robin; all
+// waves hold all T columns (U per lane) and a chunk's 64 row factors are
+// staged in the wave's own LDS rows.
+#pragma once
+
+#include "pbe_device.hpp"
+#include "pbe_internal.hpp"
+#include "agg_tile.hpp"
+
+namespace pbe {
+
+constexpr int kStripRows = 64;       // rows per chunk
+constexpr int kStripExtra = 256;     // near-birth reach of the LDS window beyond the block
+
+// Cost layout of the chunks of one launch (host and device agree on it).
+struct StripPlan {
+    long long total;   // C: total cost
+    int nb;            // column blocks
+    int nch;           // chunks of the full row range, ceil(n / 64)
+    int U;             // chunks per block height (T / 64)
+    int W;             // weight of a near chunk
+};
+
+__host__ __device__ __forceinline__ int strip_nc(const StripPlan& s, int q)
+{
+    const int c = (q + 1) * s.U;
+    return c < s.nch ? c : s.nch;
+}
+
+// First chunk of block q that is weighted W.
+__host__ __device__ __forceinline__ int strip_cb(const StripPlan& s, int q)
+{
+    const int c = (q - 1) * s.U;
+    const int nc = strip_nc(s, q);
+    return c < 0 ? 0 : (c < nc ? c : nc);
+}
+
+__host__ __device__ __forceinline__ long long strip_cost(const StripPlan& s, int q)
+{
+    const int cb = strip_cb(s, q), nc = strip_nc(s, q);
+    return (long long)cb + (long long)(nc - cb) * s.W;
+}
+
+// Cost of the blocks before q (q <= nb - 1: those blocks are all full).
+__host__ __device__ __forceinline__ long long strip_prefix(const StripPlan& s, int q)
+{
+    if (q <= 0) return 0;
+    const long long U = s.U, W = s.W, m = q - 1;
+    // block 0: U chunks of weight W; block q' in [1, q): (q'-1)U + 2UW
+    return U * W + U * m * (m - 1) / 2 + 2 * U * W * m;
+}
+
+__host__ __device__ __forceinline__ long long strip_total(const StripPlan& s)
+{
+    return strip_prefix(s, s.nb - 1) + strip_cost(s, s.nb - 1);
+}
+
+// First chunk (q, c) whose cost offset is >= t; (nb, 0) past the end.
+__device__ __forceinline__ void strip_first_unit(const StripPlan& s, long long t, int& q, int& c)
+{
+    if (t >= s.total) {
+        q = s.nb;
+        c = 0;
+        return;
+    }
+    // largest q with prefix(q) <= t: the prefix is quadratic in m = q - 1
+    const double U = s.U, W = s.W;
+    const double bq = 2.0 * U * W - 0.5 * U, cq = U * W - (double)t;
+    const double disc = bq * bq - 2.0 * U * cq;
+    int qq = (t < (long long)s.U * s.W) ? 0
+           : (int)floor((-bq + sqrt(disc > 0.0 ? disc : 0.0)) / U) + 1;
+    if (qq < 0) qq = 0;
+    if (qq > s.nb - 1) qq = s.nb - 1;
+    while (qq > 0 && strip_prefix(s, qq) > t) --qq;
+    while (qq + 1 <= s.nb - 1 && strip_prefix(s, qq + 1) <= t) ++qq;
+    const long long o = t - strip_prefix(s, qq);
+    const int cb = strip_cb(s, qq), nc = strip_nc(s, qq);
+    long long cc = (o <= cb) ? o : cb + (o - cb + s.W - 1) / s.W;
+    if (cc >= nc) {
+        q = qq + 1;
+        c = 0;
+    } else {
+        q = qq;
+        c = (int)cc;
+    }
+}
+
+// Order this wave's LDS writes before its later LDS reads by other lanes.
+// LDS instructions of one wave execute in issue order, so only code motion by
+// the compiler has to be stopped (no wait on outstanding global atomics).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_wave_barrier();
+    __asm__ __volatile__("" ::: "memory");
+}
+
+template <int KT, int U>
+struct StripSmem {
+    static constexpr int R = Kern<KT>::R;
+    static constexpr int RS = (R + 2) & ~1;
+    static constexpr int T = kWave * U;
+    static constexpr int WIN = T + kStripExtra;
+    double rowbuf[4][kStripRows * RS];   // per wave: row factors A_r(j) and x_j
+    double colwin[WIN];                  // births/deaths of the window bins
+    double xw[WIN + 1];                  // pivots of the window bins
+    double idw[WIN + 1];                 // 1/(x_{i+1} - x_i)
+    double colc[T];                      // cube roots of the block's pivots
+    double colic[T];                     // and their reciprocals
+    double tk[4][R];                     // per wave: the block's column sums TK_r
+    double tjf[4][R][kWave];             // per lane: sum of A_r over its far rows
+    double tjd[4][R][kWave];             // per lane: sum of A_r over its off-diagonal rows
+};
+
+// Column factors B_r(k) = N_k phi_r(k) of this lane's U columns.
+template <int KT, int U>
+__device__ __forceinline__ void strip_col_factors(const AggArgs& a, const StripSmem<KT, U>& sm,
+                                                  const int k0, const int lane,
+                                                  double (&Bc)[U][Kern<KT>::R])
+{
+    constexpr int R = Kern<KT>::R;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int c = lane + kWave * u;
+        const int k = k0 + c;
+        if (k < a.n) {
+            double f[R];
+            Kern<KT>::col(f, sm.xw[c], sm.colc[c], sm.colic[c]);
+            const double Nk = a.N[k];
+#pragma unroll
+            for (int r = 0; r < R; ++r) Bc[u][r] = Nk * f[r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) Bc[u][r] = 0.0;
+        }
+    }
+}
+
+// One visit of column block q by the workgroup: chunks [c_lo, c_hi).
+template <int KT, int U>
+__device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, const int c_lo,
+                                                const int c_hi, StripSmem<KT, U>& sm)
+{
+    using K = Kern<KT>;
+    constexpr int R = K::R;
+    constexpr int RS = StripSmem<KT, U>::RS;
+    constexpr int T = StripSmem<KT, U>::T;
+    constexpr int WIN = StripSmem<KT, U>::WIN;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
+    const int n = a.n;
+    const int k0 = q * T;
+    const double xlast = a.x[n - 1];
+
+    // ---- column set-up, once per visit ----
+    for (int i = tid; i < WIN; i += 256) sm.colwin[i] = 0.0;
+    for (int i = tid; i <= WIN; i += 256) {
+        const int b = k0 + i;
+        sm.xw[i] = b < n ? a.x[b] : __builtin_inf();
+        sm.idw[i] = b < n - 1 ? 1.0 / (a.x[b + 1] - a.x[b]) : 0.0;
+    }
+    for (int i = tid; i < T; i += 256) {
+        const int k = k0 + i;
+        double cb = 0.0, icb = 0.0;
+        if (k < n) pivot_roots<KT>(a.x[k], cb, icb);
+        sm.colc[i] = cb;
+        sm.colic[i] = icb;
+    }
+    __syncthreads();
+
+    double Bc[U][R], xk[U];
+    strip_col_factors<KT, U>(a, sm, k0, lane, Bc);
+#pragma unroll
+    for (int u = 0; u < U; ++u) xk[u] = sm.xw[lane + kWave * u];
+    // TK_r = sum over the block's columns of B_r(k), kept in this wave's LDS slot
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        double s = 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) s += Bc[u][r];
+        s = wave_sum(s);
+        if (lane == 0) sm.tk[wv][r] = s;
+        sm.tjf[wv][r][lane] = 0.0;
+        sm.tjd[wv][r][lane] = 0.0;
+    }
+    // far births to k+1 before the 1/(x_{k+1} - x_k) factor: sum_j (v - x_k) rate
+    double accw[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) accw[u] = 0.0;
+    double* myrows = sm.rowbuf[wv];
+    const BirthSink<WIN> sink{sm.colwin, a.rhs, k0};
+    const int rmax = min(n, k0 + T);   // rows j <= last column of the block
+    wave_lds_sync();
+
+    // This lane's row of the next chunk is loaded one chunk ahead, before the
+    // current chunk issues its global atomic: a load issued after the atomic
+    // would wait for it (vector memory counts retire in order).
+    double xj_next = 0.0, Nj_next = 0.0;
+    {
+        const int j = (c_lo + wv) * kStripRows + lane;
+        if (c_lo + wv < c_hi && j < rmax) {
+            xj_next = a.x[j];
+            Nj_next = a.N[j];
+        }
+    }
+    for (int c = c_lo + wv; c < c_hi; c += 4) {
+        const int rw0 = c * kStripRows;
+        const int rw1 = min(rw0 + kStripRows, rmax);
+        const bool diag = rw0 >= k0;
+        // row factors of this chunk: one row per lane
+        const int j = rw0 + lane;
+        const double xj = j < rw1 ? xj_next : 0.0;
+        const double Nj = Nj_next;
+        {
+            const int jn = j + 4 * kStripRows;
+            if (c + 4 < c_hi && jn < rmax) {
+                xj_next = a.x[jn];
+                Nj_next = a.N[jn];
+            }
+        }
+        double av[R];
+        if (j < rw1) {
+            double cb, icb;
+            pivot_roots<KT>(xj, cb, icb);
+            K::row(av, xj, cb, icb, a.coef);
+#pragma unroll
+            for (int r = 0; r < R; ++r) av[r] *= Nj;
+            // death of row j against the block's columns (separable)
+            double dth = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dth = fma(av[r], sm.tk[wv][r], dth);
+            if (diag) {
+                // rows == columns: bin j loses sum_{k in block} rate_jk,
+                // which also covers the column side of the block's pairs
+                lds_add(&sm.colwin[j - k0], -dth);
+            } else {
+                // issued after the next chunk's loads, so the pair loop below
+                // hides its latency
+                global_add(&a.rhs[j], -dth);
+#pragma unroll
+                for (int r = 0; r < R; ++r) sm.tjd[wv][r][lane] += av[r];
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) av[r] = 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) myrows[lane * RS + r] = av[r];
+        myrows[lane * RS + R] = xj;
+        wave_lds_sync();
+
+        const int nrows = rw1 - rw0;
+        const double xjmax = __shfl(xj, nrows - 1, kWave);
+        bool ok = !diag;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + lane + kWave * u;
+            ok = ok && (k >= n - 1 || (xjmax + xk[u]) < sm.xw[lane + kWave * u + 1]);
+        }
+        if (__all(ok)) {
+            // every pair lands in (k, k+1), or clips at column n-1 (taken as
+            // weight w = 0 there: 1/(x_{k+1} - x_k) is 0 for k = n-1)
+#pragma unroll
+            for (int r = 0; r < R; ++r) sm.tjf[wv][r][lane] += av[r];
+#pragma unroll 2
+            for (int jj = 0; jj < nrows; ++jj) {
+                const double* rp = myrows + jj * RS;
+                double ar[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) ar[r] = rp[r];
+                const double xr = rp[R];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    double pr = ar[0] * Bc[u][0];
+#pragma unroll
+                    for (int r = 1; r < R; ++r) pr = fma(ar[r], Bc[u][r], pr);
+                    const double v = xr + xk[u];
+                    const double dv = v - xk[u];
+                    accw[u] = fma(dv, pr, accw[u]);
+                }
+            }
+        } else {
+            // Pairs near the diagonal band: each pair adds its two
+            // Kumar-Ramkrishna parts to the LDS window (global atomics
+            // beyond it).  A lane keeps the exact bracket (cur, cur+1) of
+            // v = x_j + x_k for its column; rows run upward, so it only moves
+            // up, mostly by at most one bin per row.  One column at a time
+            // keeps a single bracket live.
+            auto xat = [&](int i) { return (i - k0 <= WIN) ? sm.xw[i - k0] : a.x[i]; };
+            auto idat = [&](int i) {
+                return (i - k0 <= WIN) ? sm.idw[i - k0] : 1.0 / (a.x[i + 1] - a.x[i]);
+            };
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + lane + kWave * u;
+                if (k >= n) continue;
+                int cur = k;
+                double xc = xk[u], xc1 = k < n - 1 ? sm.xw[lane + kWave * u + 1] : xk[u];
+                double idc = sm.idw[lane + kWave * u];
+                double accc = 0.0;
+                const int jend = min(nrows, k - rw0 + 1);   // rows j <= k
+                for (int jj = 0; jj < jend; ++jj) {
+                    const double* rp = myrows + jj * RS;
+                    double pr = rp[0] * Bc[u][0];
+#pragma unroll
+                    for (int r = 1; r < R; ++r) pr = fma(rp[r], Bc[u][r], pr);
+                    if (rw0 + jj == k) pr *= 0.5;
+                    const double v = rp[R] + xk[u];
+                    if (v >= xlast) {
+                        // upper clip (src/aggregation_kernels.cuh:191-192)
+                        accc += pr;
+                    } else {
+                        if (v >= xc1) {
+                            int cn = cur + 1;
+                            if (xat(cn + 1) <= v) {
+                                const int hi = (a.inv_log_r != 0.0)
+                                    ? bracket_geometric(a.x, n, v, a.log_x0, a.inv_log_r)
+                                    : bracket_search(a.x, cn, n - 1, v);
+                                cn = hi - 1;
+                            }
+                            cur = cn;
+                            xc = xat(cn);
+                            xc1 = xat(cn + 1);
+                            idc = idat(cn);
+                        }
+                        const double wp = ((v - xc) * idc) * pr;
+                        sink.put(cur, pr - wp);
+                        sink.put(cur + 1, wp);
+                    }
+                }
+                if (accc != 0.0) sink.put(n - 1, accc);
+            }
+        }
+        // the next chunk rewrites this wave's rows
+        wave_lds_sync();
+    }
+
+    // ---- this wave's column totals: far births and off-diagonal deaths ----
+    double TJF[R], TJD[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        TJF[r] = wave_sum(sm.tjf[wv][r][lane]);
+        TJD[r] = wave_sum(sm.tjd[wv][r][lane]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int c = lane + kWave * u;
+        const int k = k0 + c;
+        if (k < n) {
+            double cs = 0.0, dth = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                cs = fma(TJF[r], Bc[u][r], cs);
+                dth = fma(TJD[r], Bc[u][r], dth);
+            }
+            const double hi = accw[u] * sm.idw[c];   // 0 for k = n-1
+            const double lo = (cs - hi) - dth;
+            if (lo != 0.0) lds_add(&sm.colwin[c], lo);
+            if (hi != 0.0) lds_add(&sm.colwin[c + 1], hi);
+        }
+    }
+    __syncthreads();
+
+    // ---- flush the window ----
+    for (int i = tid; i < WIN; i += 256) {
+        const int k = k0 + i;
+        const double v = sm.colwin[i];
+        if (k < n && v != 0.0) global_add(&a.rhs[k], v);
+    }
+    __syncthreads();
+}
+
+}  // namespace pbe

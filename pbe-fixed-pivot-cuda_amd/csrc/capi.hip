@@ -26,20 +26,26 @@ static_assert(offsetof(pbe_breakage_params, block_size) == 48, "layout");
 
 namespace pbe {
 
-// Columns per lane of the aggregation tiles, from the autotuned table
-// (tile_table.inc: rows {n_max, kernel_type, tile_u}, generated on an MI355X
-// by pbe-fixed-pivot-cuda_amd/autotune.py; kernel_type -1 matches any).
-struct TileRule { int n_max, kernel, u; };
+// Columns per lane and near-chunk weight of the aggregation strips, from the
+// autotuned table (tile_table.inc: rows {n_max, kernel_type, tile_u,
+// near_weight}, generated on an MI355X by pbe-fixed-pivot-cuda_amd/autotune.py;
+// kernel_type -1 matches any).
+struct TileRule { int n_max, kernel, u, w; };
 static const TileRule kTileTable[] = {
 #include "tile_table.inc"
 };
 
-int default_tile_u(int n, int kernel_type)
+static const TileRule& tile_rule(int n, int kernel_type)
 {
+    static const TileRule fallback{1 << 30, -1, 4, 2};
     for (const TileRule& r : kTileTable)
-        if (n <= r.n_max && (r.kernel < 0 || r.kernel == kernel_type)) return r.u;
-    return 4;
+        if (n <= r.n_max && (r.kernel < 0 || r.kernel == kernel_type)) return r;
+    return fallback;
 }
+
+int default_tile_u(int n, int kernel_type) { return tile_rule(n, kernel_type).u; }
+
+int default_near_weight(int n, int kernel_type) { return tile_rule(n, kernel_type).w; }
 
 }  // namespace pbe
 
@@ -54,6 +60,7 @@ extern "C" hipError_t pbe_launch_aggregation_rhs(const double* N, const double* 
     if (p->block_size > 1024) return hipErrorInvalidConfiguration;
     hipError_t e = pbe::launch_aggregation(N, x, rhs, *p,
                                            pbe::default_tile_u(p->n, p->kernel_type),
+                                           pbe::default_near_weight(p->n, p->kernel_type),
                                            stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
@@ -70,9 +77,12 @@ extern "C" hipError_t pbe_launch_aggregation_rhs_ex(const double* N, const doubl
     if (p->kernel_type < 0 || p->kernel_type > 5) return hipErrorInvalidValue;
     if (p->block_size > 1024) return hipErrorInvalidConfiguration;
     int u = opts ? opts->tile_u : 0;
+    int w = opts ? opts->near_weight : 0;
     if (u == 0) u = pbe::default_tile_u(p->n, p->kernel_type);
+    if (w == 0) w = pbe::default_near_weight(p->n, p->kernel_type);
     if (u != 1 && u != 2 && u != 4 && u != 8) return hipErrorInvalidValue;
-    hipError_t e = pbe::launch_aggregation(N, x, rhs, *p, u, stream);
+    if (w < 1 || w > 16) return hipErrorInvalidValue;
+    hipError_t e = pbe::launch_aggregation(N, x, rhs, *p, u, w, stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }
@@ -94,6 +104,52 @@ extern "C" hipError_t pbe_launch_breakage_rhs(const double* N, const double* x,
     if (p->block_size <= 0) return hipErrorInvalidValue;
     if (p->block_size > 1024) return hipErrorInvalidConfiguration;
     hipError_t e = pbe::launch_breakage(N, x, t_q, bw_q, rhs, *p, stream);
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+// ------------------------------------------------- deterministic gather ----
+extern "C" size_t pbe_aggregation_deterministic_workspace_bytes(int n)
+{
+    if (n <= 0) return 0;
+    return pbe::det_aggregation_workspace_doubles(n) * sizeof(double);
+}
+
+extern "C" size_t pbe_breakage_deterministic_workspace_bytes(int n)
+{
+    if (n <= 0) return 0;
+    return pbe::det_breakage_workspace_doubles(n) * sizeof(double);
+}
+
+extern "C" hipError_t pbe_launch_aggregation_rhs_deterministic(
+    const double* N, const double* x, double* rhs, const pbe_aggregation_params* p,
+    void* workspace, size_t workspace_bytes, hipStream_t stream)
+{
+    if (!p || p->n <= 0) return hipErrorInvalidValue;
+    if (!N || !x || !rhs || !workspace) return hipErrorInvalidDevicePointer;
+    if (p->kernel_type < 0 || p->kernel_type > 5) return hipErrorInvalidValue;
+    if (p->block_size > 1024) return hipErrorInvalidConfiguration;
+    if (workspace_bytes < pbe_aggregation_deterministic_workspace_bytes(p->n))
+        return hipErrorInvalidValue;
+    hipError_t e = pbe::launch_aggregation_deterministic(N, x, rhs, *p,
+                                                         static_cast<double*>(workspace), stream);
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+extern "C" hipError_t pbe_launch_breakage_rhs_deterministic(
+    const double* N, const double* x, const double* t_q, const double* bw_q, double* rhs,
+    const pbe_breakage_params* p, void* workspace, size_t workspace_bytes, hipStream_t stream)
+{
+    if (!p || p->n <= 0 || p->n_quad <= 0) return hipErrorInvalidValue;
+    if (!N || !x || !t_q || !bw_q || !rhs || !workspace) return hipErrorInvalidDevicePointer;
+    if (p->selection < 0 || p->selection > 3) return hipErrorInvalidValue;
+    if (p->block_size <= 0) return hipErrorInvalidValue;
+    if (p->block_size > 1024) return hipErrorInvalidConfiguration;
+    if (workspace_bytes < pbe_breakage_deterministic_workspace_bytes(p->n))
+        return hipErrorInvalidValue;
+    hipError_t e = pbe::launch_breakage_deterministic(N, x, t_q, bw_q, rhs, *p,
+                                                      static_cast<double*>(workspace), stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }

@@ -1,0 +1,426 @@
+// det_kernels.hip -- atomic-free, bitwise repeatable aggregation and breakage
+// source terms (the gather mode; no counterpart in the reference, whose FP64
+// atomics make the summation order vary from run to run).
+//
+// Every output bin is written by exactly one lane, and every sum is formed in
+// an order fixed by n (and n_quad) alone, so two calls on the same inputs
+// return identical bits.  Each contribution is the same expression as in the
+// atomic kernels; only the order of the additions differs.
+//
+// Aggregation, three launches:
+//   1. table: per bin A_r(i) = N_i a_r(i), B_r(i) = N_i phi_r(i), x_i,
+//      1/(x_{i+1} - x_i), and per 64-bin block the column sums of B_r.
+//   2. far partials: one wave per (64-column block, row segment); lane k sums
+//      over the segment's rows j <= k whose pair is far (v = x_j + x_k <
+//      x_{k+1}, or k = n-1 where every pair clips with weight 0) the rate and
+//      the weighted rate, and writes its lo / hi parts to the segment's slot.
+//   3. gather: lane i adds, in a fixed order, the far parts of columns i and
+//      i-1 over all segments, the near pairs whose bracket is (i-1, i) or
+//      (i, i+1) (enumerated column by column with two monotone row pointers),
+//      for i = n-1 the clipped pairs (summed by its whole wave in a fixed
+//      butterfly), and the separable death -sum_r A_r(i) TB_r.
+//
+// Breakage, two launches: a table of S_j N_j (the reference's guard applied),
+// then lane i gathers its death and, per quadrature point, the parents whose
+// fragment x_j t_q lands in (x_{i-1}, x_{i+1}) (binary search on the rounded
+// fragment volume); the fragments clipped to bin 0 or bin n-1 are summed by
+// the wave that owns that bin.
+#include <algorithm>
+
+#include "pbe_device.hpp"
+#include "pbe_internal.hpp"
+
+namespace pbe {
+
+namespace {
+
+constexpr int kDetBlock = 64;   // columns per far-partial item, bins per table block
+
+template <int KT>
+struct DetLayout {
+    static constexpr int R = Kern<KT>::R;
+    static constexpr int TS = 2 * R + 2;   // table stride: A[R], B[R], x, 1/dx
+};
+
+// Row segments of the far partials: a function of n only.
+inline int det_segments(int n)
+{
+    const int s = (n + 1023) / 1024;
+    return std::min(16, std::max(1, s));
+}
+
+struct DetAggArgs {
+    const double* __restrict__ N;
+    const double* __restrict__ x;
+    double* rhs;
+    double* tab;      // [n][TS]
+    double* tbq;      // [nbq][R]
+    double* part;     // [2][S][n]: lo parts, then hi parts
+    int n, nbq, S;
+    KernelCoef coef;
+};
+
+template <int KT>
+__global__ __launch_bounds__(64) void det_agg_table_kernel(DetAggArgs a)
+{
+    using L = DetLayout<KT>;
+    constexpr int R = L::R, TS = L::TS;
+    const int i = blockIdx.x * kDetBlock + threadIdx.x;
+    double A[R], B[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        A[r] = 0.0;
+        B[r] = 0.0;
+    }
+    if (i < a.n) {
+        const double xi = a.x[i], Ni = a.N[i];
+        double cb, icb;
+        pivot_roots<KT>(xi, cb, icb);
+        double f[R];
+        Kern<KT>::row(A, xi, cb, icb, a.coef);
+        Kern<KT>::col(f, xi, cb, icb);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            A[r] *= Ni;
+            B[r] = Ni * f[r];
+        }
+        double* t = a.tab + (size_t)TS * i;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            t[r] = A[r];
+            t[R + r] = B[r];
+        }
+        t[2 * R] = xi;
+        t[2 * R + 1] = i < a.n - 1 ? 1.0 / (a.x[i + 1] - xi) : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const double s = wave_sum(B[r]);
+        if (threadIdx.x == 0) a.tbq[(size_t)R * blockIdx.x + r] = s;
+    }
+}
+
+// Far partials.  Item = (block q, segment s), heaviest blocks first.
+template <int KT>
+__global__ __launch_bounds__(64) void det_agg_far_kernel(DetAggArgs a)
+{
+    using L = DetLayout<KT>;
+    constexpr int R = L::R, TS = L::TS;
+    const int n = a.n, S = a.S;
+    const int q = a.nbq - 1 - (int)blockIdx.x / S;
+    const int s = (int)blockIdx.x % S;
+    const int k = q * kDetBlock + (int)threadIdx.x;
+    const int rq = min(n, (q + 1) * kDetBlock);   // rows j < rq (j <= k)
+    const int r0 = (int)((long long)s * rq / S), r1 = (int)((long long)(s + 1) * rq / S);
+    const bool valid = k < n;
+    double B[R], xk = 0.0, xk1 = 0.0, idk = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) B[r] = 0.0;
+    if (valid) {
+        const double* t = a.tab + (size_t)TS * k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) B[r] = t[R + r];
+        xk = t[2 * R];
+        idk = t[2 * R + 1];
+        xk1 = k < n - 1 ? a.x[k + 1] : __builtin_inf();
+    }
+    double accr = 0.0, accw = 0.0;
+    for (int j = r0; j < r1; ++j) {
+        const double* t = a.tab + (size_t)TS * j;   // the same row for every lane
+        double pr = t[0] * B[0];
+#pragma unroll
+        for (int r = 1; r < R; ++r) pr = fma(t[r], B[r], pr);
+        const double v = t[2 * R] + xk;
+        if (valid && j <= k && v < xk1) {
+            if (j == k) pr *= 0.5;
+            accr += pr;
+            accw = fma(v - xk, pr, accw);
+        }
+    }
+    if (valid) {
+        const double hi = accw * idk;   // 0 for k = n-1: every pair clips there
+        a.part[(size_t)s * n + k] = accr - hi;
+        a.part[(size_t)(S + s) * n + k] = hi;
+    }
+}
+
+// First index in [lo, hi) where pred holds (pred monotone false..true); hi if none.
+template <class P>
+__device__ __forceinline__ int first_true(int lo, int hi, P pred)
+{
+    while (lo < hi) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (pred(mid)) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+template <int KT>
+__device__ __forceinline__ double det_rate(const double* tab, int j, int k)
+{
+    using L = DetLayout<KT>;
+    constexpr int R = L::R, TS = L::TS;
+    const double* tj = tab + (size_t)TS * j;
+    const double* tk = tab + (size_t)TS * k;
+    double pr = tj[0] * tk[R];
+#pragma unroll
+    for (int r = 1; r < R; ++r) pr = fma(tj[r], tk[R + r], pr);
+    return j == k ? 0.5 * pr : pr;
+}
+
+// Near births of target i from pairs whose bracket is (Lb, Lb+1); upper = true
+// takes the weight w of the upper pivot (Lb = i - 1), else 1 - w (Lb = i).
+template <int KT>
+__device__ __forceinline__ double det_near(const DetAggArgs& a, int Lb, bool upper)
+{
+    using L = DetLayout<KT>;
+    constexpr int R = L::R, TS = L::TS;
+    const double* x = a.x;
+    const double xL = x[Lb], xL1 = x[Lb + 1];
+    const double idL = a.tab[(size_t)TS * Lb + 2 * R + 1];
+    double acc = 0.0;
+    int k = Lb - 1;
+    if (k < 0 || x[k] + x[k] < xL) return 0.0;
+    int ja = first_true(0, k + 1, [&](int j) { return x[j] + x[k] >= xL; });
+    int jb = first_true(ja, k + 1, [&](int j) { return x[j] + x[k] >= xL1; });
+    for (; k >= 0 && ja <= k; --k) {
+        const double xk = x[k];
+        while (ja <= k && x[ja] + xk < xL) ++ja;
+        if (jb < ja) jb = ja;
+        while (jb <= k && x[jb] + xk < xL1) ++jb;
+        const int je = min(jb, k + 1);   // rows j <= k
+        for (int j = ja; j < je; ++j) {
+            const double pr = det_rate<KT>(a.tab, j, k);
+            const double w = (x[j] + xk - xL) * idL;
+            acc += upper ? w * pr : pr - w * pr;
+        }
+    }
+    return acc;
+}
+
+template <int KT>
+__global__ __launch_bounds__(256) void det_agg_gather_kernel(DetAggArgs a)
+{
+    using L = DetLayout<KT>;
+    constexpr int R = L::R, TS = L::TS;
+    __shared__ double red[256];
+    __shared__ double TB[R];
+    const int n = a.n, S = a.S;
+    const int tid = threadIdx.x;
+    // TB_r = sum of all column factors, fixed order (strided partials, then a tree)
+    for (int r = 0; r < R; ++r) {
+        double s = 0.0;
+        for (int b = tid; b < a.nbq; b += 256) s += a.tbq[(size_t)R * b + r];
+        red[tid] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (tid < w) red[tid] += red[tid + w];
+            __syncthreads();
+        }
+        if (tid == 0) TB[r] = red[0];
+        __syncthreads();
+    }
+    const int i = blockIdx.x * 256 + tid;
+    const double* x = a.x;
+
+    // clipped pairs (v >= x_{n-1}, k <= n-2): the wave that owns bin n-1
+    double clip = 0.0;
+    const int wave0 = blockIdx.x * 256 + (tid & ~63);
+    if (wave0 <= n - 1 && n - 1 < wave0 + 64 && n >= 2) {
+        const double xl = x[n - 1];
+        const int lane = tid & 63;
+        for (int k = n - 2 - lane; k >= 0; k -= 64) {
+            if (x[k] + x[k] < xl) break;   // smaller k cannot reach x_{n-1}
+            const int jc = first_true(0, k + 1, [&](int j) { return x[j] + x[k] >= xl; });
+            for (int j = jc; j <= k; ++j) clip += det_rate<KT>(a.tab, j, k);
+        }
+        clip = wave_sum(clip);
+    }
+    if (i >= n) return;
+
+    double acc = 0.0;
+    for (int s = 0; s < S; ++s) acc += a.part[(size_t)s * n + i];
+    if (i >= 1)
+        for (int s = 0; s < S; ++s) acc += a.part[(size_t)(S + s) * n + i - 1];
+    if (i <= n - 2) acc += det_near<KT>(a, i, false);
+    if (i >= 1 && i - 1 <= n - 2) acc += det_near<KT>(a, i - 1, true);
+    if (i == n - 1) acc += clip;
+    const double* t = a.tab + (size_t)TS * i;
+    double dth = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dth = fma(t[r], TB[r], dth);
+    a.rhs[i] += acc - dth;
+}
+
+template <int KT>
+hipError_t det_agg_launch(const DetAggArgs& a, hipStream_t s)
+{
+    const int nbq = a.nbq;
+    hipLaunchKernelGGL(det_agg_table_kernel<KT>, dim3((unsigned)nbq), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(det_agg_far_kernel<KT>, dim3((unsigned)(nbq * a.S)), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(det_agg_gather_kernel<KT>, dim3((unsigned)((a.n + 255) / 256)), dim3(256),
+                       0, s, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------- breakage ---
+
+struct DetBrkArgs {
+    const double* __restrict__ N;
+    const double* __restrict__ x;
+    const double* __restrict__ t_q;
+    const double* __restrict__ bw_q;
+    double* rhs;
+    double* sn;       // [n]: S_j N_j of the guarded parents, else 0
+    int n, nq;
+    double S0, v_ref, alpha, v_min;
+};
+
+template <int SEL>
+__global__ __launch_bounds__(256) void det_brk_table_kernel(DetBrkArgs a)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= a.n) return;
+    const double xj = a.x[j], Nj = a.N[j];
+    const double Sj = selection<SEL>(xj, a.S0, a.v_ref, a.alpha, a.v_min);
+    // guard of src/breakage_kernels.cuh:131,195
+    a.sn[j] = (Sj != 0.0 && Nj > 2.2e-308) ? Sj * Nj : 0.0;
+}
+
+__global__ __launch_bounds__(256) void det_brk_gather_kernel(DetBrkArgs a)
+{
+    const int n = a.n, nq = a.nq;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int i = blockIdx.x * 256 + tid;
+    const double* x = a.x;
+    const double x0 = x[0], xl = x[n - 1];
+
+    // fragments clipped below x_0 (to bin 0) or at/above x_{n-1} (to bin n-1),
+    // summed by the wave that owns the bin
+    const int wave0 = blockIdx.x * 256 + (tid & ~63);
+    double clip0 = 0.0, clipn = 0.0;
+    if (wave0 == 0) {
+        for (int q = 0; q < nq; ++q) {
+            const double t = a.t_q[q], b = a.bw_q[q];
+            const int J = first_true(0, n, [&](int j) { return !(x[j] * t < x0); });
+            double s = 0.0;
+            for (int j = lane; j < J; j += 64) s += (a.sn[j] * b);
+            clip0 += s;
+        }
+        clip0 = wave_sum(clip0);
+    }
+    if (wave0 <= n - 1 && n - 1 < wave0 + 64) {
+        for (int q = 0; q < nq; ++q) {
+            const double t = a.t_q[q], b = a.bw_q[q];
+            const int J = first_true(0, n, [&](int j) { return x[j] * t >= xl; });
+            double s = 0.0;
+            for (int j = J + lane; j < n; j += 64) s += (a.sn[j] * b);
+            clipn += s;
+        }
+        clipn = wave_sum(clipn);
+    }
+    if (i >= n) return;
+
+    double acc = -a.sn[i];   // death (src/breakage_kernels.cuh:132)
+    if (n >= 2) {
+        for (int q = 0; q < nq; ++q) {
+            const double t = a.t_q[q], b = a.bw_q[q];
+            // brackets (i, i+1): weight 1 - eta; (i-1, i): weight eta; only
+            // unclipped fragments x_0 <= v < x_{n-1} (v = x_0 gives eta = 0,
+            // the same as the reference's clip to bin 0)
+            for (int side = 0; side < 2; ++side) {
+                const int lo = i - side;
+                if (lo < 0 || lo > n - 2) continue;
+                const double xlo = x[lo], xhi = x[lo + 1];
+                const int ja = first_true(0, n, [&](int j) { return x[j] * t >= xlo; });
+                const int jb = first_true(ja, n, [&](int j) { return x[j] * t >= xhi; });
+                for (int j = ja; j < jb; ++j) {
+                    const double v = x[j] * t;
+                    if (!(v < xl)) break;
+                    const double contrib = a.sn[j] * b;
+                    const double eta = (v - xlo) / (xhi - xlo);
+                    acc += side ? eta * contrib : (1.0 - eta) * contrib;
+                }
+            }
+        }
+    }
+    if (i == 0) acc += clip0;
+    if (i == n - 1) acc += clipn;
+    a.rhs[i] += acc;
+}
+
+template <int SEL>
+void det_brk_launch(const DetBrkArgs& a, hipStream_t s)
+{
+    const dim3 g((unsigned)((a.n + 255) / 256)), b(256);
+    hipLaunchKernelGGL(det_brk_table_kernel<SEL>, g, b, 0, s, a);
+    hipLaunchKernelGGL(det_brk_gather_kernel, g, b, 0, s, a);
+}
+
+}  // namespace
+
+size_t det_aggregation_workspace_doubles(int n)
+{
+    const size_t nbq = (size_t)(n + kDetBlock - 1) / kDetBlock;
+    return (size_t)n * DetLayout<5>::TS + nbq * Kern<5>::R + 2 * (size_t)det_segments(n) * n;
+}
+
+size_t det_breakage_workspace_doubles(int n) { return (size_t)n; }
+
+hipError_t launch_aggregation_deterministic(const double* N, const double* x, double* rhs,
+                                            const pbe_aggregation_params& p, double* ws,
+                                            hipStream_t stream)
+{
+    DetAggArgs a;
+    a.N = N;
+    a.x = x;
+    a.rhs = rhs;
+    a.n = p.n;
+    a.nbq = (p.n + kDetBlock - 1) / kDetBlock;
+    a.S = det_segments(p.n);
+    a.coef = KernelCoef{p.beta0, p.beta_br, p.beta_sh};
+    // the table stride depends on the kernel; the layout below fits the largest
+    const size_t tab_doubles = (size_t)p.n * DetLayout<5>::TS;
+    a.tab = ws;
+    a.tbq = ws + tab_doubles;
+    a.part = a.tbq + (size_t)a.nbq * Kern<5>::R;
+    switch (p.kernel_type) {
+    case 0: return det_agg_launch<0>(a, stream);
+    case 1: return det_agg_launch<1>(a, stream);
+    case 2: return det_agg_launch<2>(a, stream);
+    case 3: return det_agg_launch<3>(a, stream);
+    case 4: return det_agg_launch<4>(a, stream);
+    case 5: return det_agg_launch<5>(a, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_breakage_deterministic(const double* N, const double* x, const double* t_q,
+                                         const double* bw_q, double* rhs,
+                                         const pbe_breakage_params& p, double* ws,
+                                         hipStream_t stream)
+{
+    DetBrkArgs a;
+    a.N = N;
+    a.x = x;
+    a.t_q = t_q;
+    a.bw_q = bw_q;
+    a.rhs = rhs;
+    a.sn = ws;
+    a.n = p.n;
+    a.nq = p.n_quad;
+    a.S0 = p.S0;
+    a.v_ref = p.v_ref;
+    a.alpha = p.alpha;
+    a.v_min = p.v_min;
+    switch (p.selection) {
+    case 0: det_brk_launch<0>(a, stream); break;
+    case 1: det_brk_launch<1>(a, stream); break;
+    case 2: det_brk_launch<2>(a, stream); break;
+    case 3: det_brk_launch<3>(a, stream); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace pbe

@@ -9,7 +9,7 @@ namespace pbe {
 
 // Enqueue the aggregation tiles (U = columns per lane: 2, 4 or 8).
 hipError_t launch_aggregation(const double* N, const double* x, double* rhs,
-                              const pbe_aggregation_params& p, int tile_u,
+                              const pbe_aggregation_params& p, int tile_u, int near_weight,
                               hipStream_t stream);
 
 // Enqueue the fused breakage death + birth kernel.
@@ -19,6 +19,19 @@ hipError_t launch_breakage(const double* N, const double* x, const double* t_q,
 
 // Columns per lane picked for an aggregation problem of size n.
 int default_tile_u(int n, int kernel_type);
+
+// Atomic-free, bitwise repeatable forms (det_kernels.hip); ws holds
+// det_*_workspace_doubles(n) doubles.
+size_t det_aggregation_workspace_doubles(int n);
+size_t det_breakage_workspace_doubles(int n);
+hipError_t launch_aggregation_deterministic(const double* N, const double* x, double* rhs,
+                                            const pbe_aggregation_params& p, double* ws,
+                                            hipStream_t stream);
+hipError_t launch_breakage_deterministic(const double* N, const double* x, const double* t_q,
+                                         const double* bw_q, double* rhs,
+                                         const pbe_breakage_params& p, double* ws,
+                                         hipStream_t stream);
+int default_near_weight(int n, int kernel_type);
 
 // One right-hand side R(N) = R_agg(N) + R_brk(N); either part may be absent.
 // With `batch` set, R is the batched source of every instance of the plan and

@@ -51,6 +51,12 @@ class AggregationParams(ctypes.Structure):
                          float(beta0), float(beta_br), float(beta_sh), int(block_size))
 
 
+class AggOptions(ctypes.Structure):
+    """pbe_agg_options: strip width (columns per lane) and near-chunk weight;
+    0 selects the autotuned value."""
+    _fields_ = [("tile_u", ctypes.c_int), ("near_weight", ctypes.c_int)]
+
+
 class BreakageParams(ctypes.Structure):
     """Layout of BreakageParams (include/pbe_cuda/breakage.cuh:63-77)."""
     _fields_ = [("n", ctypes.c_int), ("n_quad", ctypes.c_int),
@@ -112,6 +118,38 @@ def launch_aggregation_rhs(N, x, rhs, params, stream=None):
     HIP status code, as the reference's C++ function does."""
     return library().pbe_launch_aggregation_rhs(_ptr(N), _ptr(x), _ptr(rhs),
                                                 ctypes.byref(params), _stream(stream))
+
+
+def launch_aggregation_rhs_ex(N, x, rhs, params, tile_u=0, near_weight=0, stream=None):
+    """pbe_launch_aggregation_rhs_ex with explicit launch options."""
+    opt = AggOptions(int(tile_u), int(near_weight))
+    return library().pbe_launch_aggregation_rhs_ex(_ptr(N), _ptr(x), _ptr(rhs),
+                                                   ctypes.byref(params), ctypes.byref(opt),
+                                                   _stream(stream))
+
+
+def launch_aggregation_rhs_deterministic(N, x, rhs, params, workspace, stream=None):
+    """rhs += aggregation source, atomic-free and bitwise repeatable
+    (pbe_launch_aggregation_rhs_deterministic).  `workspace` is a device array
+    of at least aggregation_deterministic_workspace_bytes(n) bytes."""
+    return library().pbe_launch_aggregation_rhs_deterministic(
+        _ptr(N), _ptr(x), _ptr(rhs), ctypes.byref(params), _ptr(workspace),
+        workspace.nbytes if workspace is not None else 0, _stream(stream))
+
+
+def launch_breakage_rhs_deterministic(N, x, t_q, bw_q, rhs, params, workspace, stream=None):
+    """rhs += breakage source, atomic-free and bitwise repeatable."""
+    return library().pbe_launch_breakage_rhs_deterministic(
+        _ptr(N), _ptr(x), _ptr(t_q), _ptr(bw_q), _ptr(rhs), ctypes.byref(params),
+        _ptr(workspace), workspace.nbytes if workspace is not None else 0, _stream(stream))
+
+
+def aggregation_deterministic_workspace_bytes(n):
+    return library().pbe_aggregation_deterministic_workspace_bytes(int(n))
+
+
+def breakage_deterministic_workspace_bytes(n):
+    return library().pbe_breakage_deterministic_workspace_bytes(int(n))
 
 
 def launch_breakage_rhs(N, x, t_q, bw_q, rhs, params, stream=None):

@@ -131,3 +131,39 @@ def test_device_cube_root_host_check():
     out = json.loads(r.stdout)
     assert r.returncode == 0, out
     assert out["cbrt_max_ulp"] <= 4 and out["inv_cbrt_max_ulp"] <= 8
+
+
+def test_deterministic_argument_errors(pbe):
+    """The gather-mode launches check arguments, then the workspace size,
+    before any HIP call."""
+    lib = pbe.library()
+    A = pbe.AggregationParams(n=100, kernel_type=5)
+    B = pbe.BreakageParams(n=100, n_quad=1)
+    need_a = lib.pbe_aggregation_deterministic_workspace_bytes(100)
+    need_b = lib.pbe_breakage_deterministic_workspace_bytes(100)
+    assert need_a > 0 and need_b > 0
+    assert lib.pbe_aggregation_deterministic_workspace_bytes(0) == 0
+    agg = lib.pbe_launch_aggregation_rhs_deterministic
+    brk = lib.pbe_launch_breakage_rhs_deterministic
+    assert agg(FAKE, FAKE, FAKE, None, FAKE, need_a, None) == INVALID_VALUE
+    assert agg(FAKE, FAKE, FAKE, ctypes.byref(A), None, need_a, None) == INVALID_DEVICE_POINTER
+    assert agg(FAKE, FAKE, FAKE, ctypes.byref(A), FAKE, need_a - 8, None) == INVALID_VALUE
+    bad = pbe.AggregationParams(n=100, kernel_type=7)
+    assert agg(FAKE, FAKE, FAKE, ctypes.byref(bad), FAKE, need_a, None) == INVALID_VALUE
+    assert brk(FAKE, FAKE, FAKE, FAKE, FAKE, ctypes.byref(B), None, need_b,
+               None) == INVALID_DEVICE_POINTER
+    assert brk(FAKE, FAKE, FAKE, FAKE, FAKE, ctypes.byref(B), FAKE, need_b - 8,
+               None) == INVALID_VALUE
+    zero = pbe.BreakageParams(n=100, n_quad=0)
+    assert brk(FAKE, FAKE, FAKE, FAKE, FAKE, ctypes.byref(zero), FAKE, need_b,
+               None) == INVALID_VALUE
+
+
+def test_aggregation_option_errors(pbe):
+    """tile_u outside {0,1,2,4,8} or near_weight outside [0,16] are refused."""
+    lib = pbe.library()
+    A = pbe.AggregationParams(n=100, kernel_type=5)
+    for u, w in ((3, 0), (16, 0), (4, 17), (4, -1)):
+        opt = pbe.AggOptions(u, w)
+        assert lib.pbe_launch_aggregation_rhs_ex(FAKE, FAKE, FAKE, ctypes.byref(A),
+                                                 ctypes.byref(opt), None) == INVALID_VALUE

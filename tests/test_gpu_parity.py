@@ -194,7 +194,6 @@ def test_flocculation_16384_vs_port(pbe, oracle_mod):
 def test_every_tile_shape(pbe, oracle_mod, u, n, kernel):
     """Each compiled tile shape (64*U square tiles) through
     pbe_launch_aggregation_rhs_ex gives the same parity."""
-    import ctypes
     from pbe_hip import hip, problems as P
     x = P.geometric_grid(n, 1e-18, 1.02)
     N = P.lognormal_ic(x, x[n // 3], 1.5, 1e12, noise=0.05, seed=u)
@@ -204,10 +203,8 @@ def test_every_tile_shape(pbe, oracle_mod, u, n, kernel):
     dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
     drhs = hip.DeviceArray(n)
     drhs.zero()
-    opt = (ctypes.c_int * 1)(u)
     p = pbe.agg_params(n, agg)
-    hip.check(pbe.library().pbe_launch_aggregation_rhs_ex(
-        dN.ptr, dx.ptr, drhs.ptr, ctypes.byref(p), ctypes.cast(opt, ctypes.c_void_p), None))
+    hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=u, near_weight=1 + u))
     hip.synchronize()
     ref, A, D = _oracle(oracle_mod, N, x, agg=agg)
     got = drhs.download()
