@@ -88,15 +88,15 @@ hipError_t pbe_launch_aggregation_rhs(const double* N, const double* x,
                                       const pbe_aggregation_params* params,
                                       hipStream_t stream);
 
-/* Launch options of the aggregation kernel (no counterpart in the reference,
+/* Launch options of the aggregation kernels (no counterpart in the reference,
  * whose only knob is the compile-time PBE_TILE, src/aggregation_kernels.cuh:
- * 30-33).  tile_u = columns per lane: column strips are 64*tile_u wide;
- * near_weight = the cost of a 64-row chunk of near-diagonal pairs relative
- * to a chunk of far pairs, used to split the work evenly over the resident
- * workgroups.  0 picks the library's autotuned choice for (n, kernel_type). */
+ * 30-33).  tile_u = columns per lane: column blocks and diagonal-band tiles
+ * are 64*tile_u wide; band_split = row slices per diagonal-band tile (each
+ * slice one workgroup).  0 picks the library's autotuned choice for
+ * (n, kernel_type). */
 typedef struct pbe_agg_options {
-    int tile_u;        /* 0, 1, 2, 4 or 8 */
-    int near_weight;   /* 0 or 1..16 */
+    int tile_u;       /* 0, 1, 2, 4 or 8 */
+    int band_split;   /* 0, 1, 2, 4 or 8 */
 } pbe_agg_options;
 
 /* pbe_launch_aggregation_rhs with explicit launch options (same results). */

@@ -2,13 +2,12 @@
 
     python pbe-fixed-pivot-cuda_amd/autotune.py --out tile_table.inc [--json results.json]
 
-For each size bucket and collision kernel, every strip width (64*U columns,
-U = 1, 2, 4, 8 columns per lane) and near-chunk weight W (the cost of a chunk
-of near-diagonal pairs relative to a far chunk, used to split the work over
-the resident workgroups) is timed with HIP events over back-to-back launches
-on the flocculation workload (lognormal N, geometric grid r = 1.02), and the
-fastest (U, W) is written as a rule {n_max, kernel, U, W} in the C include
-that the library compiles in (csrc/tile_table.inc).  The table is read-only
+For each size bucket and collision kernel, every block width (64*U columns,
+U = 1, 2, 4, 8 columns per lane) and band split S (row slices per
+diagonal-band tile, S = 1, 2, 4, 8) is timed with HIP events over
+back-to-back launches on the flocculation workload (lognormal N, geometric
+grid r = 1.02), and the fastest (U, S) is written as a rule {n_max, kernel,
+U, S} in the C include that the library compiles in (csrc/tile_table.inc).  The table is read-only
 data built offline: the library holds no tuning state at run time.
 """
 import argparse
@@ -22,7 +21,7 @@ sys.path.insert(0, os.path.join(HERE, "python"))
 SIZES = [256, 512, 1024, 2048, 4096, 8192, 16384]
 KERNELS = [0, 1, 2, 3, 4, 5]
 TILES = [1, 2, 4, 8]
-WEIGHTS = [1, 2, 3, 4, 6]
+WEIGHTS = [1, 2, 4, 8]
 
 
 def time_launch(pbe, hip, N, x, agg, u, w, reps):
@@ -34,8 +33,8 @@ def time_launch(pbe, hip, N, x, agg, u, w, reps):
     s = hip.Stream()
 
     def launch():
-        hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=u, near_weight=w,
-                                                stream=s), f"tile {u} weight {w}")
+        hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=u, band_split=w,
+                                                stream=s), f"tile {u} split {w}")
     for _ in range(3):
         launch()
     s.synchronize()
@@ -72,7 +71,7 @@ def main():
             print(f"n={n:6d} kernel={k} " + " ".join(
                 f"U{u}:{min(t[(u, w)] for w in WEIGHTS) * 1e3:8.1f}us" for u in TILES)
                 + f"  -> U={bu} W={bw} {t[(bu, bw)] * 1e3:.1f}us", flush=True)
-    lines = ["// {n_max, kernel_type (-1 = any), columns per lane, near-chunk weight};",
+    lines = ["// {n_max, kernel_type (-1 = any), columns per lane, band-tile row slices};",
              "// first match wins.  Generated by pbe-fixed-pivot-cuda_amd/autotune.py on",
              "// an MI355X (gfx950)."]
     for k in KERNELS:

@@ -7,11 +7,14 @@
 //
 // Design (MI355X-first, not the reference's one-thread-per-pair layout):
 //
-//  * Column strips of T = 64*U columns, cut into chunks of 64 rows over the
-//    upper triangle and split by an estimated cost over exactly the number
-//    of workgroups the device holds at once (agg_strip.hpp).  Lanes own
-//    columns k (U each), waves own chunks of rows; row factors are staged in
-//    LDS and read as wave-uniform broadcasts.
+//  * Two launches over the upper triangle, cut into blocks of T = 64*U
+//    columns.  The diagonal band -- for each block, the rows of the block
+//    itself and of the block below, where the near pairs live -- runs as
+//    square T x T tiles split into row slices, many small workgroups
+//    scheduled by the hardware (agg_tile.hpp).  Everything below the band
+//    runs as column strips split evenly over exactly the resident workgroups
+//    (agg_strip.hpp).  Lanes own columns k (U each), waves own rows; row
+//    factors are staged in LDS and read as wave-uniform broadcasts.
 //
 //  * The pair rate uses the separable form of the collision kernel
 //    (pbe_device.hpp): R <= 5 multiply-adds per pair.
@@ -20,29 +23,46 @@
 //    n; all of them land in the bracket (k, k+1), so a lane keeps that
 //    column's births in registers: sum_j w rate (to k+1) and the lo part
 //    sum_j (1-w) rate = colsum_k - sum_j w rate, where colsum_k is the
-//    separable column sum of the rates.  A chunk whose rows are all far from
-//    the wave's columns runs a 3-op-plus-rate inner loop with no compares.
+//    separable column sum of the rates.  A chunk of rows all far from the
+//    wave's columns runs a 3-op-plus-rate inner loop with no compares.
 //    Column n-1 joins them: all its pairs clip to bin n-1 (weight w = 0).
 //
-//  * "Near" pairs (diagonal band) keep a monotone exact bracket per column
-//    (a cursor) and ds_add_f64 the per-bracket sums into an LDS window of the
-//    block's target bins (global atomics beyond it).
+//  * "Near" pairs (the band) keep a monotone exact bracket per column (a
+//    cursor) and ds_add_f64 their births into an LDS window of the tile's
+//    target bins (global atomics beyond it).
 //
-//  * Death N_i sum_k beta_ik N_k is separable: a row chunk below the block
-//    loses sum_r A_r(j) TK_r (TK_r: the block's column sums) and the block's
-//    columns lose sum_r B_r(k) TJ_r; rows inside the block lose
-//    sum_r A_r(i) TK_r, which covers both sides of the block's own pairs.
+//  * Death N_i sum_k beta_ik N_k is separable: rows lose sum_r A_r(j) TK_r
+//    (TK_r: the column sums of the tile or block) and columns lose
+//    sum_r B_r(k) TJ_r; a diagonal tile's rows lose sum_r A_r(i) TK_r, which
+//    covers both sides of its own pairs.
 //
-//  * One flush per visit of a block: the LDS window is added to rhs with
-//    FP64 global atomics (zero entries skipped).
+//  * One flush per tile or visit of a block: the LDS window is added to rhs
+//    with FP64 global atomics (zero entries skipped).
 #include <atomic>
 
 #include "agg_strip.hpp"
 
 namespace pbe {
 
-// At least 3 waves per SIMD (3 workgroups per CU, VGPRs capped at 168), 2
-// for the widest strips.
+// Diagonal band: the 2 nb - 1 tiles with d = 0 (diagonal) and d = 1 (the
+// block below), each split into S row slices; item t -> tile t / S, slice
+// t % S, band tiles in order p = 0.. for d = 0, then for d = 1.
+template <int KT, int U>
+__global__ __launch_bounds__(256)
+void agg_band_kernel(AggArgs a, int S)
+{
+    __shared__ AggSmem<KT, U> sm;
+    constexpr int T = AggShape<U>::T, RW = AggShape<U>::RW;
+    const int t = blockIdx.x;
+    const int tt = t / S, sl = t - tt * S;
+    const int d = tt < a.nb ? 0 : 1;
+    const int p = tt < a.nb ? tt : tt - a.nb;
+    agg_tile_body<KT, U>(a, p * T, (p + d) * T, d == 0, p * T + sl * (T / S), RW / S, sm);
+}
+
+// Rows below the band, as column strips over the resident grid.  At least 3
+// waves per SIMD (3 workgroups per CU, VGPRs capped at 168), 2 for the
+// widest strips.
 template <int KT, int U>
 __global__ __launch_bounds__(256, (U >= 8 ? 2 : 3))
 void agg_strip_kernel(AggArgs a, StripPlan plan)
@@ -58,7 +78,7 @@ void agg_strip_kernel(AggArgs a, StripPlan plan)
     }
 }
 
-// Workgroups of one kernel resident on the current device at once.
+// Workgroups of the strip kernel resident on the current device at once.
 template <int KT, int U>
 static int resident_groups()
 {
@@ -81,38 +101,43 @@ static int resident_groups()
 }
 
 template <int KT, int U>
-static hipError_t launch_agg_t(const AggArgs& args, int near_weight, hipStream_t stream)
+static hipError_t launch_agg_t(AggArgs args, int split, hipStream_t stream)
 {
+    constexpr int T = AggShape<U>::T;
+    args.nb = (args.n + T - 1) / T;
+    const long long band_items = (long long)split * band_tiles(args.nb);
+    if (band_items > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((agg_band_kernel<KT, U>), dim3((unsigned)band_items), dim3(256), 0,
+                       stream, args, split);
     StripPlan plan;
     plan.U = U;
-    plan.W = near_weight;
-    plan.nb = (args.n + 64 * U - 1) / (64 * U);
-    plan.nch = (args.n + kStripRows - 1) / kStripRows;
+    plan.nb = args.nb;
     plan.total = strip_total(plan);
-    long long G = resident_groups<KT, U>();
-    if (G > plan.total) G = plan.total;
-    if (G < 1) G = 1;
-    hipLaunchKernelGGL((agg_strip_kernel<KT, U>), dim3((unsigned)G), dim3(256), 0, stream,
-                       args, plan);
+    if (plan.total > 0) {
+        long long G = resident_groups<KT, U>();
+        if (G > plan.total) G = plan.total;
+        hipLaunchKernelGGL((agg_strip_kernel<KT, U>), dim3((unsigned)G), dim3(256), 0, stream,
+                           args, plan);
+    }
     return hipSuccess;
 }
 
 template <int U>
-static hipError_t launch_agg_u(const AggArgs& args, int kernel, int w, hipStream_t stream)
+static hipError_t launch_agg_u(const AggArgs& args, int kernel, int split, hipStream_t stream)
 {
     switch (kernel) {
-    case 0: return launch_agg_t<0, U>(args, w, stream);
-    case 1: return launch_agg_t<1, U>(args, w, stream);
-    case 2: return launch_agg_t<2, U>(args, w, stream);
-    case 3: return launch_agg_t<3, U>(args, w, stream);
-    case 4: return launch_agg_t<4, U>(args, w, stream);
-    case 5: return launch_agg_t<5, U>(args, w, stream);
+    case 0: return launch_agg_t<0, U>(args, split, stream);
+    case 1: return launch_agg_t<1, U>(args, split, stream);
+    case 2: return launch_agg_t<2, U>(args, split, stream);
+    case 3: return launch_agg_t<3, U>(args, split, stream);
+    case 4: return launch_agg_t<4, U>(args, split, stream);
+    case 5: return launch_agg_t<5, U>(args, split, stream);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_aggregation(const double* N, const double* x, double* rhs,
-                              const pbe_aggregation_params& p, int tile_u, int near_weight,
+                              const pbe_aggregation_params& p, int tile_u, int band_split,
                               hipStream_t stream)
 {
     AggArgs args;
@@ -120,16 +145,16 @@ hipError_t launch_aggregation(const double* N, const double* x, double* rhs,
     args.x = x;
     args.rhs = rhs;
     args.n = p.n;
+    args.nb = 0;
     args.coef = KernelCoef{p.beta0, p.beta_br, p.beta_sh};
     args.log_x0 = p.log_x0;
     args.inv_log_r = p.inv_log_r;
-    args.nb = 0;
-    const int w = near_weight < 1 ? 1 : near_weight;
+    const int S = (band_split == 1 || band_split == 2 || band_split == 8) ? band_split : 4;
     switch (tile_u) {
-    case 1: return launch_agg_u<1>(args, p.kernel_type, w, stream);
-    case 2: return launch_agg_u<2>(args, p.kernel_type, w, stream);
-    case 8: return launch_agg_u<8>(args, p.kernel_type, w, stream);
-    default: return launch_agg_u<4>(args, p.kernel_type, w, stream);
+    case 1: return launch_agg_u<1>(args, p.kernel_type, S, stream);
+    case 2: return launch_agg_u<2>(args, p.kernel_type, S, stream);
+    case 8: return launch_agg_u<8>(args, p.kernel_type, S, stream);
+    default: return launch_agg_u<4>(args, p.kernel_type, S, stream);
     }
 }
 

@@ -1,22 +1,23 @@
-// agg_strip.hpp -- single-instance aggregation as column strips split evenly
-// over a resident grid (device code; the launcher is in agg_kernels.hip).
+// agg_strip.hpp -- single-instance aggregation, far-pair part: column strips
+// split evenly over a resident grid (device code; the launcher and the
+// diagonal-band part are in agg_kernels.hip).
 //
 // Work decomposition.  Column block q holds the T = 64*U columns
-// [qT, qT + T); its pairs are the rows j <= k, cut into chunks of 64 rows.
-// The chunks are laid out block after block, rows ascending within a block,
-// and weighted by an estimate of their cost: 1 for a chunk whose rows all lie
-// at least one block below the block (far pairs), W for the two blocks of
-// rows at and just below the diagonal (near pairs, cursor path).  The grid is
-// exactly the number of workgroups the device holds at once, and workgroup g
-// takes the chunks whose cumulative cost starts in [g C / G, (g+1) C / G):
-// every workgroup is resident from the start and all finish together, and
-// a workgroup usually stays within one or two column blocks, so the column
-// set-up (pivots, cube roots, the LDS window) and the flush of the window are
-// paid once per visit of a block rather than once per 64 x T tile.
+// [qT, qT + T).  Its rows j < (q-1)T -- everything below the block's
+// diagonal band, which the band kernel takes -- are cut into chunks of 64
+// rows.  The chunks are laid out block after block, rows ascending within a
+// block; on a grid whose near-pair reach is below T they are all far chunks
+// of equal cost.  The grid is exactly the number of workgroups the device
+// holds at once, and workgroup g takes chunks [g C / G, (g+1) C / G): every
+// workgroup is resident from the start and all finish together, and a
+// workgroup stays within one or two column blocks, so the column set-up
+// (pivots, cube roots, the LDS window) and the flush of the window are paid
+// once per visit of a block rather than once per tile.
 //
 // Within a visit the four waves take the visit's chunks round robin; all
 // waves hold all T columns (U per lane) and a chunk's 64 row factors are
-// staged in the wave's own LDS rows.
+// staged in the wave's own LDS rows.  A chunk that is not entirely far (a
+// grid finer than the band) takes a per-pair path with LDS atomics.
 #pragma once
 
 #include "pbe_device.hpp"
@@ -28,50 +29,32 @@ namespace pbe {
 constexpr int kStripRows = 64;       // rows per chunk
 constexpr int kStripExtra = 256;     // near-birth reach of the LDS window beyond the block
 
-// Cost layout of the chunks of one launch (host and device agree on it).
+// Chunk layout of one launch: block q >= 2 has (q - 1) U chunks.
 struct StripPlan {
-    long long total;   // C: total cost
+    long long total;   // C: number of chunks
     int nb;            // column blocks
-    int nch;           // chunks of the full row range, ceil(n / 64)
     int U;             // chunks per block height (T / 64)
-    int W;             // weight of a near chunk
 };
 
 __host__ __device__ __forceinline__ int strip_nc(const StripPlan& s, int q)
 {
-    const int c = (q + 1) * s.U;
-    return c < s.nch ? c : s.nch;
+    return q >= 2 ? (q - 1) * s.U : 0;
 }
 
-// First chunk of block q that is weighted W.
-__host__ __device__ __forceinline__ int strip_cb(const StripPlan& s, int q)
-{
-    const int c = (q - 1) * s.U;
-    const int nc = strip_nc(s, q);
-    return c < 0 ? 0 : (c < nc ? c : nc);
-}
-
-__host__ __device__ __forceinline__ long long strip_cost(const StripPlan& s, int q)
-{
-    const int cb = strip_cb(s, q), nc = strip_nc(s, q);
-    return (long long)cb + (long long)(nc - cb) * s.W;
-}
-
-// Cost of the blocks before q (q <= nb - 1: those blocks are all full).
+// Chunks of the blocks before q.
 __host__ __device__ __forceinline__ long long strip_prefix(const StripPlan& s, int q)
 {
-    if (q <= 0) return 0;
-    const long long U = s.U, W = s.W, m = q - 1;
-    // block 0: U chunks of weight W; block q' in [1, q): (q'-1)U + 2UW
-    return U * W + U * m * (m - 1) / 2 + 2 * U * W * m;
+    if (q <= 2) return 0;
+    const long long m = q - 2;
+    return (long long)s.U * m * (m + 1) / 2;
 }
 
 __host__ __device__ __forceinline__ long long strip_total(const StripPlan& s)
 {
-    return strip_prefix(s, s.nb - 1) + strip_cost(s, s.nb - 1);
+    return strip_prefix(s, s.nb);
 }
 
-// First chunk (q, c) whose cost offset is >= t; (nb, 0) past the end.
+// Chunk (q, c) at linear index t; (nb, 0) past the end.
 __device__ __forceinline__ void strip_first_unit(const StripPlan& s, long long t, int& q, int& c)
 {
     if (t >= s.total) {
@@ -79,26 +62,13 @@ __device__ __forceinline__ void strip_first_unit(const StripPlan& s, long long t
         c = 0;
         return;
     }
-    // largest q with prefix(q) <= t: the prefix is quadratic in m = q - 1
-    const double U = s.U, W = s.W;
-    const double bq = 2.0 * U * W - 0.5 * U, cq = U * W - (double)t;
-    const double disc = bq * bq - 2.0 * U * cq;
-    int qq = (t < (long long)s.U * s.W) ? 0
-           : (int)floor((-bq + sqrt(disc > 0.0 ? disc : 0.0)) / U) + 1;
-    if (qq < 0) qq = 0;
-    if (qq > s.nb - 1) qq = s.nb - 1;
-    while (qq > 0 && strip_prefix(s, qq) > t) --qq;
-    while (qq + 1 <= s.nb - 1 && strip_prefix(s, qq + 1) <= t) ++qq;
-    const long long o = t - strip_prefix(s, qq);
-    const int cb = strip_cb(s, qq), nc = strip_nc(s, qq);
-    long long cc = (o <= cb) ? o : cb + (o - cb + s.W - 1) / s.W;
-    if (cc >= nc) {
-        q = qq + 1;
-        c = 0;
-    } else {
-        q = qq;
-        c = (int)cc;
-    }
+    // q = m + 2 with U m (m + 1) / 2 <= t < U (m + 1)(m + 2) / 2
+    int m = (int)floor(0.5 * (sqrt(1.0 + 8.0 * (double)t / s.U) - 1.0));
+    if (m < 0) m = 0;
+    while (m > 0 && strip_prefix(s, m + 2) > t) --m;
+    while (strip_prefix(s, m + 3) <= t) ++m;
+    q = m + 2;
+    c = (int)(t - strip_prefix(s, q));
 }
 
 // Order this wave's LDS writes before its later LDS reads by other lanes.
@@ -276,6 +246,8 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
             // weight w = 0 there: 1/(x_{k+1} - x_k) is 0 for k = n-1)
 #pragma unroll
             for (int r = 0; r < R; ++r) sm.tjf[wv][r][lane] += av[r];
+            // the U rate chains are written interleaved (r outer) so that
+            // consecutive FP64 instructions are independent
 #pragma unroll 2
             for (int jj = 0; jj < nrows; ++jj) {
                 const double* rp = myrows + jj * RS;
@@ -283,15 +255,17 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
 #pragma unroll
                 for (int r = 0; r < R; ++r) ar[r] = rp[r];
                 const double xr = rp[R];
+                double pr[U], dv[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    double pr = ar[0] * Bc[u][0];
+                for (int u = 0; u < U; ++u) pr[u] = ar[0] * Bc[u][0];
 #pragma unroll
-                    for (int r = 1; r < R; ++r) pr = fma(ar[r], Bc[u][r], pr);
-                    const double v = xr + xk[u];
-                    const double dv = v - xk[u];
-                    accw[u] = fma(dv, pr, accw[u]);
-                }
+                for (int r = 1; r < R; ++r)
+#pragma unroll
+                    for (int u = 0; u < U; ++u) pr[u] = fma(ar[r], Bc[u][r], pr[u]);
+#pragma unroll
+                for (int u = 0; u < U; ++u) dv[u] = (xr + xk[u]) - xk[u];
+#pragma unroll
+                for (int u = 0; u < U; ++u) accw[u] = fma(dv[u], pr[u], accw[u]);
             }
         } else {
             // Pairs near the diagonal band: each pair adds its two

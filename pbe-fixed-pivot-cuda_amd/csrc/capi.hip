@@ -26,9 +26,9 @@ static_assert(offsetof(pbe_breakage_params, block_size) == 48, "layout");
 
 namespace pbe {
 
-// Columns per lane and near-chunk weight of the aggregation strips, from the
-// autotuned table (tile_table.inc: rows {n_max, kernel_type, tile_u,
-// near_weight}, generated on an MI355X by pbe-fixed-pivot-cuda_amd/autotune.py;
+// Columns per lane and band-tile row slices of the aggregation kernels, from
+// the autotuned table (tile_table.inc: rows {n_max, kernel_type, tile_u,
+// band_split}, generated on an MI355X by pbe-fixed-pivot-cuda_amd/autotune.py;
 // kernel_type -1 matches any).
 struct TileRule { int n_max, kernel, u, w; };
 static const TileRule kTileTable[] = {
@@ -37,7 +37,7 @@ static const TileRule kTileTable[] = {
 
 static const TileRule& tile_rule(int n, int kernel_type)
 {
-    static const TileRule fallback{1 << 30, -1, 4, 2};
+    static const TileRule fallback{1 << 30, -1, 4, 4};
     for (const TileRule& r : kTileTable)
         if (n <= r.n_max && (r.kernel < 0 || r.kernel == kernel_type)) return r;
     return fallback;
@@ -45,7 +45,7 @@ static const TileRule& tile_rule(int n, int kernel_type)
 
 int default_tile_u(int n, int kernel_type) { return tile_rule(n, kernel_type).u; }
 
-int default_near_weight(int n, int kernel_type) { return tile_rule(n, kernel_type).w; }
+int default_band_split(int n, int kernel_type) { return tile_rule(n, kernel_type).w; }
 
 }  // namespace pbe
 
@@ -60,7 +60,7 @@ extern "C" hipError_t pbe_launch_aggregation_rhs(const double* N, const double* 
     if (p->block_size > 1024) return hipErrorInvalidConfiguration;
     hipError_t e = pbe::launch_aggregation(N, x, rhs, *p,
                                            pbe::default_tile_u(p->n, p->kernel_type),
-                                           pbe::default_near_weight(p->n, p->kernel_type),
+                                           pbe::default_band_split(p->n, p->kernel_type),
                                            stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
@@ -77,11 +77,11 @@ extern "C" hipError_t pbe_launch_aggregation_rhs_ex(const double* N, const doubl
     if (p->kernel_type < 0 || p->kernel_type > 5) return hipErrorInvalidValue;
     if (p->block_size > 1024) return hipErrorInvalidConfiguration;
     int u = opts ? opts->tile_u : 0;
-    int w = opts ? opts->near_weight : 0;
+    int w = opts ? opts->band_split : 0;
     if (u == 0) u = pbe::default_tile_u(p->n, p->kernel_type);
-    if (w == 0) w = pbe::default_near_weight(p->n, p->kernel_type);
+    if (w == 0) w = pbe::default_band_split(p->n, p->kernel_type);
     if (u != 1 && u != 2 && u != 4 && u != 8) return hipErrorInvalidValue;
-    if (w < 1 || w > 16) return hipErrorInvalidValue;
+    if (w != 1 && w != 2 && w != 4 && w != 8) return hipErrorInvalidValue;
     hipError_t e = pbe::launch_aggregation(N, x, rhs, *p, u, w, stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();

@@ -73,7 +73,7 @@ hipError_t enqueue_rhs(const RhsProblem& p, const double* Nin, double* rhs, hipS
     if (p.agg) {
         e = launch_aggregation(Nin, p.x, rhs, *p.agg,
                                default_tile_u(p.agg->n, p.agg->kernel_type),
-                               default_near_weight(p.agg->n, p.agg->kernel_type), s);
+                               default_band_split(p.agg->n, p.agg->kernel_type), s);
         if (e != hipSuccess) return e;
     }
     if (p.brk) {

@@ -9,7 +9,7 @@ namespace pbe {
 
 // Enqueue the aggregation tiles (U = columns per lane: 2, 4 or 8).
 hipError_t launch_aggregation(const double* N, const double* x, double* rhs,
-                              const pbe_aggregation_params& p, int tile_u, int near_weight,
+                              const pbe_aggregation_params& p, int tile_u, int band_split,
                               hipStream_t stream);
 
 // Enqueue the fused breakage death + birth kernel.
@@ -31,7 +31,7 @@ hipError_t launch_breakage_deterministic(const double* N, const double* x, const
                                          const double* bw_q, double* rhs,
                                          const pbe_breakage_params& p, double* ws,
                                          hipStream_t stream);
-int default_near_weight(int n, int kernel_type);
+int default_band_split(int n, int kernel_type);
 
 // One right-hand side R(N) = R_agg(N) + R_brk(N); either part may be absent.
 // With `batch` set, R is the batched source of every instance of the plan and

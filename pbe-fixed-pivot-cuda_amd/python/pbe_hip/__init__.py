@@ -52,9 +52,9 @@ class AggregationParams(ctypes.Structure):
 
 
 class AggOptions(ctypes.Structure):
-    """pbe_agg_options: strip width (columns per lane) and near-chunk weight;
-    0 selects the autotuned value."""
-    _fields_ = [("tile_u", ctypes.c_int), ("near_weight", ctypes.c_int)]
+    """pbe_agg_options: block width (columns per lane) and row slices per
+    diagonal-band tile; 0 selects the autotuned value."""
+    _fields_ = [("tile_u", ctypes.c_int), ("band_split", ctypes.c_int)]
 
 
 class BreakageParams(ctypes.Structure):
@@ -120,9 +120,9 @@ def launch_aggregation_rhs(N, x, rhs, params, stream=None):
                                                 ctypes.byref(params), _stream(stream))
 
 
-def launch_aggregation_rhs_ex(N, x, rhs, params, tile_u=0, near_weight=0, stream=None):
+def launch_aggregation_rhs_ex(N, x, rhs, params, tile_u=0, band_split=0, stream=None):
     """pbe_launch_aggregation_rhs_ex with explicit launch options."""
-    opt = AggOptions(int(tile_u), int(near_weight))
+    opt = AggOptions(int(tile_u), int(band_split))
     return library().pbe_launch_aggregation_rhs_ex(_ptr(N), _ptr(x), _ptr(rhs),
                                                    ctypes.byref(params), ctypes.byref(opt),
                                                    _stream(stream))

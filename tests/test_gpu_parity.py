@@ -204,7 +204,7 @@ def test_every_tile_shape(pbe, oracle_mod, u, n, kernel):
     drhs = hip.DeviceArray(n)
     drhs.zero()
     p = pbe.agg_params(n, agg)
-    hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=u, near_weight=1 + u))
+    hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=u, band_split=8 // u))
     hip.synchronize()
     ref, A, D = _oracle(oracle_mod, N, x, agg=agg)
     got = drhs.download()
