@@ -48,7 +48,7 @@ namespace pbe {
 // block below), each split into S row slices; item t -> tile t / S, slice
 // t % S, band tiles in order p = 0.. for d = 0, then for d = 1.
 template <int KT, int U>
-__global__ __launch_bounds__(256)
+__global__ __launch_bounds__(256, (U >= 8 ? 2 : 4))
 void agg_band_kernel(AggArgs a, int S)
 {
     __shared__ AggSmem<KT, U> sm;

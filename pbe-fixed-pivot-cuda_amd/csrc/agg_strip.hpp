@@ -89,9 +89,10 @@ struct StripSmem {
     double rowbuf[4][kStripRows * RS];   // per wave: row factors A_r(j) and x_j
     double colwin[WIN];                  // births/deaths of the window bins
     double xw[WIN + 1];                  // pivots of the window bins
-    double idw[WIN + 1];                 // 1/(x_{i+1} - x_i)
+    double idw[T];                       // 1/(x_{i+1} - x_i) of the block's columns
     double colc[T];                      // cube roots of the block's pivots
     double colic[T];                     // and their reciprocals
+    double coln[T];                      // populations of the block's columns
     double tk[4][R];                     // per wave: the block's column sums TK_r
     double tjf[4][R][kWave];             // per lane: sum of A_r over its far rows
     double tjd[4][R][kWave];             // per lane: sum of A_r over its off-diagonal rows
@@ -111,7 +112,7 @@ __device__ __forceinline__ void strip_col_factors(const AggArgs& a, const StripS
         if (k < a.n) {
             double f[R];
             Kern<KT>::col(f, sm.xw[c], sm.colc[c], sm.colic[c]);
-            const double Nk = a.N[k];
+            const double Nk = sm.coln[c];
 #pragma unroll
             for (int r = 0; r < R; ++r) Bc[u][r] = Nk * f[r];
         } else {
@@ -136,19 +137,40 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
     const int k0 = q * T;
     const double xlast = a.x[n - 1];
 
-    // ---- column set-up, once per visit ----
-    for (int i = tid; i < WIN; i += 256) sm.colwin[i] = 0.0;
-    for (int i = tid; i <= WIN; i += 256) {
-        const int b = k0 + i;
-        sm.xw[i] = b < n ? a.x[b] : __builtin_inf();
-        sm.idw[i] = b < n - 1 ? 1.0 / (a.x[b + 1] - a.x[b]) : 0.0;
+    // ---- column set-up, once per visit: all global loads first ----
+    constexpr int NW = (WIN + 1 + 255) / 256;
+    constexpr int NT = (T + 255) / 256;
+    double xv[NW], nv[NT], xn[NT];
+#pragma unroll
+    for (int t = 0; t < NW; ++t) {
+        const int i = tid + 256 * t, b = k0 + i;
+        xv[t] = (i <= WIN && b < n) ? a.x[b] : __builtin_inf();
     }
-    for (int i = tid; i < T; i += 256) {
-        const int k = k0 + i;
-        double cb = 0.0, icb = 0.0;
-        if (k < n) pivot_roots<KT>(a.x[k], cb, icb);
-        sm.colc[i] = cb;
-        sm.colic[i] = icb;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int i = tid + 256 * t, k = k0 + i;
+        nv[t] = (i < T && k < n) ? a.N[k] : 0.0;
+        xn[t] = (i < T && k + 1 < n) ? a.x[k + 1] : __builtin_inf();
+    }
+    for (int i = tid; i < WIN; i += 256) sm.colwin[i] = 0.0;
+#pragma unroll
+    for (int t = 0; t < NW; ++t) {
+        const int i = tid + 256 * t;
+        if (i <= WIN) sm.xw[i] = xv[t];
+    }
+    // the block's columns: 1/(x_{k+1} - x_k) (the near path forms the rest of
+    // the window's when it needs them), populations, cube roots
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int i = tid + 256 * t, k = k0 + i;
+        if (i < T) {
+            sm.idw[i] = k < n - 1 ? 1.0 / (xn[t] - xv[t]) : 0.0;
+            sm.coln[i] = nv[t];
+            double cb = 0.0, icb = 0.0;
+            if (k < n) pivot_roots<KT>(xv[t], cb, icb);
+            sm.colc[i] = cb;
+            sm.colic[i] = icb;
+        }
     }
     __syncthreads();
 
@@ -246,27 +268,55 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
             // weight w = 0 there: 1/(x_{k+1} - x_k) is 0 for k = n-1)
 #pragma unroll
             for (int r = 0; r < R; ++r) sm.tjf[wv][r][lane] += av[r];
-            // the U rate chains are written interleaved (r outer) so that
-            // consecutive FP64 instructions are independent
-#pragma unroll 2
-            for (int jj = 0; jj < nrows; ++jj) {
-                const double* rp = myrows + jj * RS;
-                double ar[R];
+            // Two rows at a time with their own accumulators: 2U independent
+            // FP64 chains (FP64 results take many cycles to come back, so a
+            // wave needs about eight in flight); the r-outer order keeps
+            // consecutive instructions independent.
+            double accw2[U];
 #pragma unroll
-                for (int r = 0; r < R; ++r) ar[r] = rp[r];
-                const double xr = rp[R];
-                double pr[U], dv[U];
+            for (int u = 0; u < U; ++u) accw2[u] = 0.0;
+            int jj = 0;
+            for (; jj + 1 < nrows; jj += 2) {
+                const double* r0 = myrows + jj * RS;
+                const double* r1 = r0 + RS;
+                double a0[R], a1[R];
 #pragma unroll
-                for (int u = 0; u < U; ++u) pr[u] = ar[0] * Bc[u][0];
+                for (int r = 0; r < R; ++r) {
+                    a0[r] = r0[r];
+                    a1[r] = r1[r];
+                }
+                const double x0 = r0[R], x1 = r1[R];
+                double p0[U], p1[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    p0[u] = a0[0] * Bc[u][0];
+                    p1[u] = a1[0] * Bc[u][0];
+                }
 #pragma unroll
                 for (int r = 1; r < R; ++r)
 #pragma unroll
-                    for (int u = 0; u < U; ++u) pr[u] = fma(ar[r], Bc[u][r], pr[u]);
+                    for (int u = 0; u < U; ++u) {
+                        p0[u] = fma(a0[r], Bc[u][r], p0[u]);
+                        p1[u] = fma(a1[r], Bc[u][r], p1[u]);
+                    }
 #pragma unroll
-                for (int u = 0; u < U; ++u) dv[u] = (xr + xk[u]) - xk[u];
-#pragma unroll
-                for (int u = 0; u < U; ++u) accw[u] = fma(dv[u], pr[u], accw[u]);
+                for (int u = 0; u < U; ++u) {
+                    accw[u] = fma((x0 + xk[u]) - xk[u], p0[u], accw[u]);
+                    accw2[u] = fma((x1 + xk[u]) - xk[u], p1[u], accw2[u]);
+                }
             }
+            if (jj < nrows) {
+                const double* r0 = myrows + jj * RS;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    double p = r0[0] * Bc[u][0];
+#pragma unroll
+                    for (int r = 1; r < R; ++r) p = fma(r0[r], Bc[u][r], p);
+                    accw[u] = fma((r0[R] + xk[u]) - xk[u], p, accw[u]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) accw[u] += accw2[u];
         } else {
             // Pairs near the diagonal band: each pair adds its two
             // Kumar-Ramkrishna parts to the LDS window (global atomics
@@ -276,7 +326,9 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
             // keeps a single bracket live.
             auto xat = [&](int i) { return (i - k0 <= WIN) ? sm.xw[i - k0] : a.x[i]; };
             auto idat = [&](int i) {
-                return (i - k0 <= WIN) ? sm.idw[i - k0] : 1.0 / (a.x[i + 1] - a.x[i]);
+                const int o = i - k0;
+                return o < T ? sm.idw[o]
+                     : (o < WIN ? 1.0 / (sm.xw[o + 1] - sm.xw[o]) : 1.0 / (a.x[i + 1] - a.x[i]));
             };
 #pragma unroll
             for (int u = 0; u < U; ++u) {

@@ -62,7 +62,8 @@ struct AggSmem {
     double rowbuf[4][AggShape<U>::RW * RS];
     double colwin[AggShape<U>::WIN];
     double xw[AggShape<U>::WIN + 1];    // pivots of the window bins
-    double idw[AggShape<U>::WIN + 1];   // 1/(x_{i+1} - x_i) of the window bins
+    double idw[AggShape<U>::T];         // 1/(x_{i+1} - x_i) of the tile's columns
+    double coln[AggShape<U>::T];        // populations of the tile's columns
     double rowwin[AggShape<U>::T];      // row deaths of an off-diagonal tile
     double colc[AggShape<U>::T];        // cube roots of the tile's column pivots
     double colic[AggShape<U>::T];       // and their reciprocals
@@ -106,25 +107,55 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
     const int wv = tid >> 6;
     const int n = a.n;
 
-    for (int i = tid; i < WIN; i += 256) colwin[i] = 0.0;
-    for (int i = tid; i < AggShape<U>::T; i += 256) sm.rowwin[i] = 0.0;
-    for (int i = tid; i <= WIN; i += 256) {
-        const int b = k0 + i;
-        sm.xw[i] = b < n ? a.x[b] : __builtin_inf();
-        sm.idw[i] = b < n - 1 ? 1.0 / (a.x[b + 1] - a.x[b]) : 0.0;
+    constexpr int T = AggShape<U>::T;
+    constexpr int NW = (WIN + 1 + 255) / 256;   // window pivots per thread
+    constexpr int NT = (T + 255) / 256;         // tile columns per thread
+    const int rw0 = row_base + wv * rpw;
+    const int rw1 = min(rw0 + rpw, n);
+
+    // ---- one load phase: window pivots, column populations, first rows ----
+    double xv[NW], nv[NT], xn[NT];
+#pragma unroll
+    for (int t = 0; t < NW; ++t) {
+        const int i = tid + 256 * t, b = k0 + i;
+        xv[t] = (i <= WIN && b < n) ? a.x[b] : __builtin_inf();
     }
-    // cube roots of the column pivots, once per workgroup
-    for (int i = tid; i < AggShape<U>::T; i += 256) {
-        const int k = k0 + i;
-        double cb = 0.0, icb = 0.0;
-        if (k < n) pivot_roots<KT>(a.x[k], cb, icb);
-        sm.colc[i] = cb;
-        sm.colic[i] = icb;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int i = tid + 256 * t, k = k0 + i;
+        nv[t] = (i < T && k < n) ? a.N[k] : 0.0;
+        xn[t] = (i < T && k + 1 < n) ? a.x[k + 1] : __builtin_inf();
+    }
+    double xr0 = 0.0, nr0 = 0.0;
+    if (lane < rpw && rw0 + lane < rw1) {
+        xr0 = a.x[rw0 + lane];
+        nr0 = a.N[rw0 + lane];
+    }
+    const double xlast = a.x[n - 1];
+
+    for (int i = tid; i < WIN; i += 256) colwin[i] = 0.0;
+    for (int i = tid; i < T; i += 256) sm.rowwin[i] = 0.0;
+#pragma unroll
+    for (int t = 0; t < NW; ++t) {
+        const int i = tid + 256 * t;
+        if (i <= WIN) sm.xw[i] = xv[t];
+    }
+    // column tables: 1/(x_{k+1} - x_k), populations, cube roots
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int i = tid + 256 * t, k = k0 + i;
+        if (i < T) {
+            const double xk0 = xv[t];
+            sm.idw[i] = k < n - 1 ? 1.0 / (xn[t] - xk0) : 0.0;
+            sm.coln[i] = nv[t];
+            double cb = 0.0, icb = 0.0;
+            if (k < n) pivot_roots<KT>(xk0, cb, icb);
+            sm.colc[i] = cb;
+            sm.colic[i] = icb;
+        }
     }
 
     // ---- row factors of this wave's rows -> LDS ----
-    const int rw0 = row_base + wv * rpw;
-    const int rw1 = min(rw0 + rpw, n);
     double* myrows = rowbuf[wv];
     double tj[R];
 #pragma unroll
@@ -133,9 +164,9 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
         const int j = rw0 + rr;
         double av[R];
         double xj = 0.0;
-        if (j < n) {
-            xj = a.x[j];
-            const double Nj = a.N[j];
+        if (j < rw1) {
+            xj = rr == lane ? xr0 : a.x[j];
+            const double Nj = rr == lane ? nr0 : a.N[j];
             double cb, icb;
             pivot_roots<KT>(xj, cb, icb);
             K::row(av, xj, cb, icb, a.coef);
@@ -155,7 +186,6 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
 #pragma unroll
     for (int r = 0; r < R; ++r) tj[r] = wave_sum(tj[r]);
 
-    const double xlast = a.x[n - 1];
     __syncthreads();
 
     // ---- column factors of this lane's columns (from the shared tables) ----
@@ -168,7 +198,7 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
         kc[u] = k;
         if (k < n) {
             const double xv = sm.xw[c];
-            const double Nk = a.N[k];
+            const double Nk = sm.coln[c];
             double f[R];
             K::col(f, xv, sm.colc[c], sm.colic[c]);
 #pragma unroll
@@ -293,7 +323,9 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
             const BirthSink<WIN> sink{colwin, a.rhs, k0};
             auto xat = [&](int i) { return (i - k0 <= WIN) ? sm.xw[i - k0] : a.x[i]; };
             auto idat = [&](int i) {
-                return (i - k0 <= WIN) ? sm.idw[i - k0] : 1.0 / (a.x[i + 1] - a.x[i]);
+                const int o = i - k0;
+                return o < T ? sm.idw[o]
+                     : (o < WIN ? 1.0 / (sm.xw[o + 1] - sm.xw[o]) : 1.0 / (a.x[i + 1] - a.x[i]));
             };
             // One column at a time (u outer) keeps a single column's cursor
             // state live, so this path needs few more registers than the

@@ -22,12 +22,34 @@ __device__ __forceinline__ void global_add(double* p, double v)
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Butterfly sum over the 64 lanes of a wave; every lane gets the total.
+// Move a double across lanes with a DPP control (both 32-bit halves); lanes
+// whose row is not in ROW_MASK read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes of a wave in a fixed order; every lane gets the
+// total.  Data-parallel-primitive moves instead of LDS permutes: pairs
+// (quad_perm 1,0,3,2), quads (2,3,0,1), half rows (row_half_mirror), rows of
+// 16 (row_mirror), then row 0 into row 1 and row 2 into row 3
+// (row_bcast15), rows 0-1 into row 3 (row_bcast31); lane 63 holds the total.
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
-    return v;
+    v += dpp_move<0xB1, 0xF>(v);
+    v += dpp_move<0x4E, 0xF>(v);
+    v += dpp_move<0x141, 0xF>(v);
+    v += dpp_move<0x140, 0xF>(v);
+    v += dpp_move<0x142, 0xA>(v);
+    v += dpp_move<0x143, 0xC>(v);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 // Unique right bracket hi with x[hi-1] <= v < x[hi], for x[0] < v < x[n-1]
