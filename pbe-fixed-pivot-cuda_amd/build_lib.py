@@ -84,12 +84,17 @@ def build_host_checks():
 
 
 def build_bench_helpers():
-    """bench/batched_loop.cpp: batched launch vs a loop of plain calls."""
+    """bench/batched_loop.cpp: batched launch vs a loop of plain calls;
+    bench/fp64_issue.hip: FP64 FMA issue rates (waves per SIMD x chains)."""
     src = os.path.join(HERE, "bench", "batched_loop.cpp")
     out = os.path.join(HERE, "bench", "batched_loop")
     _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", src, "-o", out + ".tmp",
           f"-L{LIBDIR}", "-lpbe_hip", "-Wl,-rpath,$ORIGIN/../lib"])
     os.replace(out + ".tmp", out)
+    src = os.path.join(HERE, "bench", "fp64_issue.hip")
+    fp = os.path.join(HERE, "bench", "fp64_issue")
+    _run([HIPCC, *COMMON, src, "-o", fp + ".tmp"])
+    os.replace(fp + ".tmp", fp)
     return out
 
 

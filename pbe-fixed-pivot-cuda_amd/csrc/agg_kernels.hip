@@ -48,16 +48,41 @@ namespace pbe {
 // block below), each split into S row slices; item t -> tile t / S, slice
 // t % S, band tiles in order p = 0.. for d = 0, then for d = 1.
 template <int KT, int U>
-__global__ __launch_bounds__(256, (U >= 8 ? 2 : 4))
-void agg_band_kernel(AggArgs a, int S)
+__device__ __forceinline__ void band_item(const AggArgs& a, int t, int S, AggSmem<KT, U>& sm)
 {
-    __shared__ AggSmem<KT, U> sm;
     constexpr int T = AggShape<U>::T, RW = AggShape<U>::RW;
-    const int t = blockIdx.x;
     const int tt = t / S, sl = t - tt * S;
     const int d = tt < a.nb ? 0 : 1;
     const int p = tt < a.nb ? tt : tt - a.nb;
     agg_tile_body<KT, U>(a, p * T, (p + d) * T, d == 0, p * T + sl * (T / S), RW / S, sm);
+}
+
+template <int KT, int U>
+__global__ __launch_bounds__(256, (U >= 8 ? 2 : 3))
+void agg_band_kernel(AggArgs a, int S)
+{
+    __shared__ AggSmem<KT, U> sm;
+    band_item<KT, U>(a, blockIdx.x, S, sm);
+}
+
+// Small problems (too few far chunks to fill the device once): one launch of
+// every tile, the band slices first, then the far tiles whole in
+// diagonal-major order, all scheduled by the hardware.
+template <int KT, int U>
+__global__ __launch_bounds__(256, (U >= 8 ? 2 : 3))
+void agg_tiles_kernel(AggArgs a, int S)
+{
+    __shared__ AggSmem<KT, U> sm;
+    constexpr int T = AggShape<U>::T, RW = AggShape<U>::RW;
+    const long long bt = band_tiles(a.nb);
+    const long long t = blockIdx.x;
+    if (t < S * bt) {
+        band_item<KT, U>(a, (int)t, S, sm);
+    } else {
+        int p, d;
+        decode_tile(t - S * bt + bt, a.nb, p, d);
+        agg_tile_body<KT, U>(a, p * T, (p + d) * T, d == 0, p * T, RW, sm);
+    }
 }
 
 // Rows below the band, as column strips over the resident grid.  At least 3
@@ -106,19 +131,25 @@ static hipError_t launch_agg_t(AggArgs args, int split, hipStream_t stream)
     constexpr int T = AggShape<U>::T;
     args.nb = (args.n + T - 1) / T;
     const long long band_items = (long long)split * band_tiles(args.nb);
-    if (band_items > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((agg_band_kernel<KT, U>), dim3((unsigned)band_items), dim3(256), 0,
-                       stream, args, split);
     StripPlan plan;
     plan.U = U;
     plan.nb = args.nb;
     plan.total = strip_total(plan);
-    if (plan.total > 0) {
-        long long G = resident_groups<KT, U>();
-        if (G > plan.total) G = plan.total;
-        hipLaunchKernelGGL((agg_strip_kernel<KT, U>), dim3((unsigned)G), dim3(256), 0, stream,
-                           args, plan);
+    const long long G = resident_groups<KT, U>();
+    if (plan.total < G) {
+        // too little far work for the strips: one launch of all tiles
+        const long long tiles = (long long)args.nb * (args.nb + 1) / 2;
+        const long long items = band_items + tiles - band_tiles(args.nb);
+        if (items > 0x7fffffffLL) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((agg_tiles_kernel<KT, U>), dim3((unsigned)items), dim3(256), 0,
+                           stream, args, split);
+        return hipSuccess;
     }
+    if (band_items > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((agg_band_kernel<KT, U>), dim3((unsigned)band_items), dim3(256), 0,
+                       stream, args, split);
+    hipLaunchKernelGGL((agg_strip_kernel<KT, U>), dim3((unsigned)G), dim3(256), 0, stream,
+                       args, plan);
     return hipSuccess;
 }
 

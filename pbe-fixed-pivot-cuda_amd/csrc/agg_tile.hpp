@@ -55,14 +55,16 @@ struct BirthSink {
     }
 };
 
+constexpr int kCursorRows = 4;   // rows whose rates the cursor path forms together
+
 template <int KT, int U>
 struct AggSmem {
     static constexpr int R = Kern<KT>::R;
     static constexpr int RS = (R + 2) & ~1;   // row record: R factors + x_j, 16-B aligned
-    double rowbuf[4][AggShape<U>::RW * RS];
+    double rowbuf[4][(AggShape<U>::RW + kCursorRows) * RS];
     double colwin[AggShape<U>::WIN];
     double xw[AggShape<U>::WIN + 1];    // pivots of the window bins
-    double idw[AggShape<U>::T];         // 1/(x_{i+1} - x_i) of the tile's columns
+    double idw[AggShape<U>::WIN];       // 1/(x_{i+1} - x_i) of the window bins
     double coln[AggShape<U>::T];        // populations of the tile's columns
     double rowwin[AggShape<U>::T];      // row deaths of an off-diagonal tile
     double colc[AggShape<U>::T];        // cube roots of the tile's column pivots
@@ -114,17 +116,17 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
     const int rw1 = min(rw0 + rpw, n);
 
     // ---- one load phase: window pivots, column populations, first rows ----
-    double xv[NW], nv[NT], xn[NT];
+    double xv[NW], xn[NW], nv[NT];
 #pragma unroll
     for (int t = 0; t < NW; ++t) {
         const int i = tid + 256 * t, b = k0 + i;
         xv[t] = (i <= WIN && b < n) ? a.x[b] : __builtin_inf();
+        xn[t] = (i < WIN && b + 1 < n) ? a.x[b + 1] : __builtin_inf();
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int i = tid + 256 * t, k = k0 + i;
         nv[t] = (i < T && k < n) ? a.N[k] : 0.0;
-        xn[t] = (i < T && k + 1 < n) ? a.x[k + 1] : __builtin_inf();
     }
     double xr0 = 0.0, nr0 = 0.0;
     if (lane < rpw && rw0 + lane < rw1) {
@@ -139,14 +141,15 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
     for (int t = 0; t < NW; ++t) {
         const int i = tid + 256 * t;
         if (i <= WIN) sm.xw[i] = xv[t];
+        // 1/(x_{i+1} - x_i) of every window bin (the near pairs' weights)
+        if (i < WIN) sm.idw[i] = k0 + i < n - 1 ? 1.0 / (xn[t] - xv[t]) : 0.0;
     }
-    // column tables: 1/(x_{k+1} - x_k), populations, cube roots
+    // column tables: populations, cube roots
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int i = tid + 256 * t, k = k0 + i;
         if (i < T) {
             const double xk0 = xv[t];
-            sm.idw[i] = k < n - 1 ? 1.0 / (xn[t] - xk0) : 0.0;
             sm.coln[i] = nv[t];
             double cb = 0.0, icb = 0.0;
             if (k < n) pivot_roots<KT>(xk0, cb, icb);
@@ -324,9 +327,28 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
             auto xat = [&](int i) { return (i - k0 <= WIN) ? sm.xw[i - k0] : a.x[i]; };
             auto idat = [&](int i) {
                 const int o = i - k0;
-                return o < T ? sm.idw[o]
-                     : (o < WIN ? 1.0 / (sm.xw[o + 1] - sm.xw[o]) : 1.0 / (a.x[i + 1] - a.x[i]));
+                return o < WIN ? sm.idw[o] : 1.0 / (a.x[i + 1] - a.x[i]);
             };
+            // Bracket of v = x_j + x_k beyond (k, k+1): a binary search in the
+            // LDS copy of the window's pivots, or over all pivots past it.
+            auto jump = [&](int c, double v) {
+                if (v < sm.xw[WIN]) return k0 + window_bracket_lo(sm.xw, c - k0, WIN, v);
+                const int hi = (a.inv_log_r != 0.0)
+                    ? bracket_geometric(a.x, n, v, a.log_x0, a.inv_log_r)
+                    : bracket_search(a.x, c, n - 1, v);
+                return hi - 1;
+            };
+            // Each column's bracket at the wave's first row, all columns
+            // together (their searches are independent).
+            int cur0[U];
+            const double xfirst = myrows[R];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = kc[u];
+                cur0[u] = k;
+                const double v0 = xfirst + xk[u];
+                if (k < n && rw0 <= k && v0 < xlast && v0 >= xk1[u]) cur0[u] = jump(k + 1, v0);
+            }
             // One column at a time (u outer) keeps a single column's cursor
             // state live, so this path needs few more registers than the
             // far path above.
@@ -334,18 +356,30 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
             for (int u = 0; u < U; ++u) {
                 const int k = kc[u];
                 if (k >= n) continue;
-                int cur = k;
-                double xc = xk[u], xc1 = xk1[u], idc = idx[u];
+                int cur = cur0[u];
+                double xc = cur == k ? xk[u] : xat(cur);
+                double xc1 = cur == k ? xk1[u] : xat(cur + 1);
+                double idc = cur == k ? idx[u] : idat(cur);
                 double accr = 0.0, accw = 0.0, accc = 0.0;
                 const int jend = min(nrows, k - rw0 + 1);   // rows j <= k
-                for (int jj = 0; jj < jend; ++jj) {
-                    const int j = rw0 + jj;
-                    const double* rp = myrows + jj * RS;
-                    double pr = rp[0] * Bc[u][0];
+                // Rates of four rows at a time first (four independent FP64
+                // chains), then the bracket logic for them in row order.
+                for (int jb = 0; jb < jend; jb += kCursorRows) {
+                double prb[kCursorRows], vb[kCursorRows];
 #pragma unroll
-                    for (int r = 1; r < R; ++r) pr = fma(rp[r], Bc[u][r], pr);
-                    if (j == k) pr *= 0.5;
-                    const double v = rp[R] + xk[u];
+                for (int q = 0; q < kCursorRows; ++q) {
+                    const double* rp = myrows + (jb + q) * RS;   // rows past nrows hold zeros
+                    double p = rp[0] * Bc[u][0];
+#pragma unroll
+                    for (int r = 1; r < R; ++r) p = fma(rp[r], Bc[u][r], p);
+                    prb[q] = (rw0 + jb + q == k) ? 0.5 * p : p;
+                    vb[q] = rp[R] + xk[u];
+                }
+#pragma unroll
+                for (int q = 0; q < kCursorRows; ++q) {
+                    if (jb + q >= jend) break;
+                    const double pr = prb[q];
+                    const double v = vb[q];
                     if (v >= xlast) {
                         // upper clip (src/aggregation_kernels.cuh:191-192)
                         accc += pr;
@@ -356,12 +390,7 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
                                 sink.put(cur + 1, accw);
                             }
                             int c = cur + 1;
-                            if (xat(c + 1) <= v) {
-                                const int hi = (a.inv_log_r != 0.0)
-                                    ? bracket_geometric(a.x, n, v, a.log_x0, a.inv_log_r)
-                                    : bracket_search(a.x, c, n - 1, v);
-                                c = hi - 1;
-                            }
+                            if (xat(c + 1) <= v) c = jump(c, v);
                             cur = c;
                             xc = xat(c);
                             xc1 = xat(c + 1);
@@ -372,6 +401,7 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
                         accr += pr;
                         accw = fma((v - xc) * idc, pr, accw);
                     }
+                }
                 }
                 if (accr != 0.0 || accw != 0.0) {
                     sink.put(cur, accr - accw);

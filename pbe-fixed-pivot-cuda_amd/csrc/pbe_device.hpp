@@ -66,6 +66,17 @@ __device__ __forceinline__ int bracket_search(const double* __restrict__ x,
     return hi;
 }
 
+// Largest i in [lo, hi) with xw[i] <= v, for xw[lo] <= v < xw[hi] (an LDS
+// copy of consecutive pivots).
+__device__ __forceinline__ int window_bracket_lo(const double* xw, int lo, int hi, double v)
+{
+    while (hi - lo > 1) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (xw[mid] <= v) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
 // Geometric-grid bracket: the reference's O(1) log-ratio guess
 // (src/aggregation_kernels.cuh:121-124) followed by a fix-up against the real
 // pivots, so the result is the same unique bracket as the search above even

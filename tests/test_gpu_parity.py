@@ -210,3 +210,18 @@ def test_every_tile_shape(pbe, oracle_mod, u, n, kernel):
     got = drhs.download()
     assert_parity(got, ref, A, what=f"U={u} n={n}")
     assert_m1(got, x, A, D, what=f"U={u} n={n}")
+
+
+def test_strips_jittered_grid(pbe, oracle_mod):
+    """Large non-geometric grid (r = 1.01 +-20 %): the column-strip launch with
+    the binary-search bracket and near pairs below the diagonal band."""
+    from pbe_hip import problems as P
+    n = 12000
+    x = P.jittered_grid(n, 1e-18, 1.01, 0.2, seed=12000)
+    N = P.lognormal_ic(x, x[n // 3], 3.0, 1e12, noise=0.05, seed=13)
+    agg = dict(kernel=5, beta0=0.0, beta_br=6.73e-18, beta_sh=1.0e-3, log_x0=0.0,
+               inv_log_r=0.0)
+    got = _gpu(pbe, N, x, agg=agg)
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg)
+    assert_parity(got, ref, A, what="strips jittered n=12000")
+    assert_m1(got, x, A, D, what="strips jittered n=12000")
