@@ -96,7 +96,7 @@ hipError_t pbe_launch_aggregation_rhs(const double* N, const double* x,
  * (n, kernel_type). */
 typedef struct pbe_agg_options {
     int tile_u;       /* 0, 1, 2, 4 or 8 */
-    int band_split;   /* 0, 1, 2, 4 or 8 */
+    int band_split;   /* 0, 1, 2, 4, 8 or 16 */
 } pbe_agg_options;
 
 /* pbe_launch_aggregation_rhs with explicit launch options (same results). */

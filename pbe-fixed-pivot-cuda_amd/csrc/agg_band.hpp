@@ -1,0 +1,285 @@
+// agg_band.hpp -- the diagonal band of one column block as a diagonal sweep
+// (device code; the launcher is in agg_kernels.hip).
+//
+// The band of column block q (T = 256 columns [k0, k0 + T)) holds its pairs
+// with rows j in [max(0, k0 - T), k]: every near pair of a grid whose
+// near-pair reach is below T, plus the far pairs in between.  Lane = column
+// k (one per thread, 256 threads).  The pairs of a column are walked by their
+// diagonal offset e = k - j from large to small, so v = x_j + x_k grows and
+// the lane's bracket (cur, cur + 1) only moves up -- and since neighbouring
+// columns step through the same offsets together, on a geometric grid their
+// brackets move at the same step: the bracket update is a uniform branch
+// rather than one taken by a few lanes every row.
+//
+// A work item is one block and one segment of offsets; the segment is walked
+// as four interleaved streams (four independent FP64 chains per lane).  Row
+// factors A_r(j) and x_j of the band's rows sit in LDS as separate arrays, so
+// a lane's gather of row j = k - e reads consecutive addresses across the
+// wave.  Per-bracket sums are added to an LDS window of target bins, then
+// flushed with global atomics.  The item of segment 0 also adds the band's
+// deaths (separable, as in the tile body).
+#pragma once
+
+#include "pbe_device.hpp"
+#include "pbe_internal.hpp"
+#include "agg_tile.hpp"
+
+namespace pbe {
+
+constexpr int kBandT = 256;               // columns per block (one per thread)
+constexpr int kBandRows = 2 * kBandT;     // rows of the band: block below + block
+constexpr int kBandWin = kBandT + 256;    // target bins held in LDS
+constexpr int kBandStreams = 4;
+
+template <int KT>
+struct BandSmem {
+    static constexpr int R = Kern<KT>::R;
+    double ra[R + 1][kBandRows];   // A_r(j) of the band rows, then x_j
+    double colwin[kBandWin];
+    double xw[kBandWin + 1];
+    double idw[kBandWin];
+    double red[4][2 * R];          // per-wave TK_r (block columns), TJ_r (rows below the block)
+};
+
+// One (block, offset segment) item.  nseg segments split the offsets
+// [0, 2T) evenly; every thread calls it.
+template <int KT>
+__device__ __forceinline__ void agg_band_sweep(const AggArgs& a, const int q, const int seg,
+                                               const int nseg, BandSmem<KT>& sm)
+{
+    using K = Kern<KT>;
+    constexpr int R = K::R;
+    constexpr int T = kBandT, WIN = kBandWin;
+    constexpr int NS = kBandStreams;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
+    const int n = a.n;
+    const int k0 = q * T;
+    const int jlo = k0 - T > 0 ? k0 - T : 0;          // first band row
+    const int nrows = min(n, k0 + T) - jlo;            // band rows (j <= last column)
+    const int k = k0 + tid;
+    const bool col = k < n;
+
+    // ---- load phase: window pivots, band rows, this thread's column ----
+    constexpr int NW = (WIN + 1 + 255) / 256;
+    double xv[NW], xn[NW];
+#pragma unroll
+    for (int t = 0; t < NW; ++t) {
+        const int i = tid + 256 * t, b = k0 + i;
+        xv[t] = (i <= WIN && b < n) ? a.x[b] : __builtin_inf();
+        xn[t] = (i < WIN && b + 1 < n) ? a.x[b + 1] : __builtin_inf();
+    }
+    double xr[2], nr[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int i = tid + 256 * t;
+        xr[t] = i < nrows ? a.x[jlo + i] : 0.0;
+        nr[t] = i < nrows ? a.N[jlo + i] : 0.0;
+    }
+    const double nk = col ? a.N[k] : 0.0;
+    const double xlast = a.x[n - 1];
+
+    for (int i = tid; i < WIN; i += 256) sm.colwin[i] = 0.0;
+#pragma unroll
+    for (int t = 0; t < NW; ++t) {
+        const int i = tid + 256 * t;
+        if (i <= WIN) sm.xw[i] = xv[t];
+        if (i < WIN) sm.idw[i] = k0 + i < n - 1 ? 1.0 / (xn[t] - xv[t]) : 0.0;
+    }
+    // row factors A_r(j) of the band rows; partial TJ over the rows below the block
+    double tj[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) tj[r] = 0.0;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int i = tid + 256 * t;
+        double av[R];
+        if (i < nrows) {
+            double cb, icb;
+            pivot_roots<KT>(xr[t], cb, icb);
+            K::row(av, xr[t], cb, icb, a.coef);
+#pragma unroll
+            for (int r = 0; r < R; ++r) av[r] *= nr[t];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) av[r] = 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            sm.ra[r][i] = av[r];
+            if (jlo + i < k0) tj[r] += av[r];
+        }
+        sm.ra[R][i] = xr[t];
+    }
+    // this thread's column factors B_r(k)
+    double B[R];
+    const double xk = xv[0];
+    {
+        double cb = 0.0, icb = 0.0;
+        if (col) pivot_roots<KT>(xk, cb, icb);
+        double f[R];
+        K::col(f, xk, cb, icb);
+#pragma unroll
+        for (int r = 0; r < R; ++r) B[r] = col ? nk * f[r] : 0.0;
+    }
+    if (seg == 0) {
+        // block totals TK_r = sum_k B_r(k), TJ_r = sum over rows below the block
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const double s1 = wave_sum(B[r]), s2 = wave_sum(tj[r]);
+            if (lane == 0) {
+                sm.red[wv][r] = s1;
+                sm.red[wv][R + r] = s2;
+            }
+        }
+    }
+    __syncthreads();
+
+    const double xk1 = sm.xw[tid + 1];   // x_{k+1} (inf past the end)
+    auto xat = [&](int i) { return (i - k0 <= WIN) ? lds_ld(&sm.xw[i - k0]) : glb_ld(&a.x[i]); };
+    auto idat = [&](int i) {
+        const int o = i - k0;
+        return o < WIN ? lds_ld(&sm.idw[o]) : 1.0 / (glb_ld(&a.x[i + 1]) - glb_ld(&a.x[i]));
+    };
+    auto put = [&](int i, double c) {
+        const int o = i - k0;
+        if (o >= 0 && o < WIN) lds_add(&sm.colwin[o], c);
+        else global_add(&a.rhs[i], c);
+    };
+    auto jump = [&](int c, double v) {
+        if (v < sm.xw[WIN]) return k0 + window_bracket_lo(sm.xw, c - k0, WIN, v);
+        const int hi = (a.inv_log_r != 0.0)
+            ? bracket_geometric(a.x, n, v, a.log_x0, a.inv_log_r)
+            : bracket_search(a.x, c, n - 1, v);
+        return hi - 1;
+    };
+
+    // ---- deaths of the band (segment 0 only) ----
+    if (seg == 0) {
+        double TK[R], TJ[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            TK[r] = (sm.red[0][r] + sm.red[1][r]) + (sm.red[2][r] + sm.red[3][r]);
+            TJ[r] = (sm.red[0][R + r] + sm.red[1][R + r]) + (sm.red[2][R + r] + sm.red[3][R + r]);
+        }
+        // rows: -sum_r A_r(j) TK_r; a row of the block itself also covers the
+        // column side of the block's own pairs
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int i = tid + 256 * t;
+            if (i < nrows) {
+                double dth = 0.0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) dth = fma(sm.ra[r][i], TK[r], dth);
+                put(jlo + i, -dth);
+            }
+        }
+        // columns: -sum_r B_r(k) TJ_r for the pairs with the rows below the block
+        if (col && jlo < k0) {
+            double dth = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dth = fma(B[r], TJ[r], dth);
+            lds_add(&sm.colwin[tid], -dth);
+        }
+    }
+
+    // ---- births: the segment's offsets as four interleaved streams ----
+    const int emax = k - jlo;                       // largest offset of this column
+    const int e0 = seg * kBandRows / nseg, e1 = (seg + 1) * kBandRows / nseg;
+    const int L = (e1 - e0 + NS - 1) / NS;          // steps per stream
+    // Each stream's bracket at its first pair, all four searches together:
+    // a stream's first pair is where its bracket is furthest from (k, k+1),
+    // and in the corner of block 0 every step starts a new column.
+    int cur[NS];
+    double xc[NS], xc1[NS], idc[NS], accr[NS], accw[NS], accc[NS];
+    {
+        int lo[NS], hi[NS];
+        double vs[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int es = min(e0 + (s + 1) * L - 1, min(e1 - 1, emax));
+            const bool valid = col && es >= e0 + s * L;
+            vs[s] = valid ? sm.ra[R][k - es - jlo] + xk : 0.0;
+            lo[s] = tid;
+            // brackets beyond the window are left for the first step's search
+            hi[s] = (valid && vs[s] >= xk1 && vs[s] < xlast && vs[s] < sm.xw[WIN]) ? WIN : tid + 1;
+        }
+        for (int step = 0; step < 10; ++step) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                if (hi[s] - lo[s] > 1) {
+                    const int mid = lo[s] + ((hi[s] - lo[s]) >> 1);
+                    if (sm.xw[mid] <= vs[s]) lo[s] = mid; else hi[s] = mid;
+                }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            cur[s] = k0 + lo[s];
+            xc[s] = sm.xw[lo[s]];
+            xc1[s] = sm.xw[lo[s] + 1];
+            idc[s] = sm.idw[lo[s]];
+            accr[s] = 0.0;
+            accw[s] = 0.0;
+            accc[s] = 0.0;
+        }
+    }
+    for (int it = 0; it < L; ++it) {
+        double pr[NS], vv[NS];
+        bool ok[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int e = e0 + (s + 1) * L - 1 - it;   // offsets run downward
+            ok[s] = col && e < e1 && e <= emax;
+            const int rr = ok[s] ? k - e - jlo : 0;
+            double p = sm.ra[0][rr] * B[0];
+#pragma unroll
+            for (int r = 1; r < R; ++r) p = fma(sm.ra[r][rr], B[r], p);
+            pr[s] = e == 0 ? 0.5 * p : p;
+            vv[s] = sm.ra[R][rr] + xk;
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            if (!ok[s]) continue;
+            const double v = vv[s];
+            if (v >= xlast) {
+                // upper clip (src/aggregation_kernels.cuh:191-192)
+                accc[s] += pr[s];
+                continue;
+            }
+            if (v >= xc1[s]) {
+                if (accr[s] != 0.0 || accw[s] != 0.0) {
+                    put(cur[s], accr[s] - accw[s]);
+                    put(cur[s] + 1, accw[s]);
+                }
+                int c = cur[s] + 1;
+                if (xat(c + 1) <= v) c = jump(c, v);
+                cur[s] = c;
+                xc[s] = xat(c);
+                xc1[s] = xat(c + 1);
+                idc[s] = idat(c);
+                accr[s] = 0.0;
+                accw[s] = 0.0;
+            }
+            accr[s] += pr[s];
+            accw[s] = fma((v - xc[s]) * idc[s], pr[s], accw[s]);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        if (accr[s] != 0.0 || accw[s] != 0.0) {
+            put(cur[s], accr[s] - accw[s]);
+            put(cur[s] + 1, accw[s]);
+        }
+        if (accc[s] != 0.0) put(n - 1, accc[s]);
+    }
+    __syncthreads();
+
+    // ---- flush the window ----
+    for (int i = tid; i < WIN; i += 256) {
+        const int b = k0 + i;
+        const double v = sm.colwin[i];
+        if (b < n && v != 0.0) global_add(&a.rhs[b], v);
+    }
+}
+
+}  // namespace pbe

@@ -40,6 +40,7 @@
 //    with FP64 global atomics (zero entries skipped).
 #include <atomic>
 
+#include "agg_band.hpp"
 #include "agg_strip.hpp"
 
 namespace pbe {
@@ -63,6 +64,16 @@ void agg_band_kernel(AggArgs a, int S)
 {
     __shared__ AggSmem<KT, U> sm;
     band_item<KT, U>(a, blockIdx.x, S, sm);
+}
+
+// Diagonal band of 256-column blocks as diagonal sweeps (agg_band.hpp):
+// item = (block, offset segment).
+template <int KT>
+__global__ __launch_bounds__(256, 3)
+void agg_sweep_kernel(AggArgs a, int nseg)
+{
+    __shared__ BandSmem<KT> sm;
+    agg_band_sweep<KT>(a, blockIdx.x / nseg, blockIdx.x % nseg, nseg, sm);
 }
 
 // Small problems (too few far chunks to fill the device once): one launch of
@@ -146,8 +157,14 @@ static hipError_t launch_agg_t(AggArgs args, int split, hipStream_t stream)
         return hipSuccess;
     }
     if (band_items > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((agg_band_kernel<KT, U>), dim3((unsigned)band_items), dim3(256), 0,
-                       stream, args, split);
+    if (T == kBandT) {
+        // band as diagonal sweeps: split = offset segments per block
+        hipLaunchKernelGGL((agg_sweep_kernel<KT>), dim3((unsigned)(args.nb * split)), dim3(256),
+                           0, stream, args, split);
+    } else {
+        hipLaunchKernelGGL((agg_band_kernel<KT, U>), dim3((unsigned)band_items), dim3(256), 0,
+                           stream, args, split);
+    }
     hipLaunchKernelGGL((agg_strip_kernel<KT, U>), dim3((unsigned)G), dim3(256), 0, stream,
                        args, plan);
     return hipSuccess;
@@ -180,7 +197,8 @@ hipError_t launch_aggregation(const double* N, const double* x, double* rhs,
     args.coef = KernelCoef{p.beta0, p.beta_br, p.beta_sh};
     args.log_x0 = p.log_x0;
     args.inv_log_r = p.inv_log_r;
-    const int S = (band_split == 1 || band_split == 2 || band_split == 8) ? band_split : 4;
+    const int S = (band_split == 1 || band_split == 2 || band_split == 8 || band_split == 16)
+        ? band_split : 4;
     switch (tile_u) {
     case 1: return launch_agg_u<1>(args, p.kernel_type, S, stream);
     case 2: return launch_agg_u<2>(args, p.kernel_type, S, stream);

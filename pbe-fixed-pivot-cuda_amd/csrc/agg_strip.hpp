@@ -94,8 +94,6 @@ struct StripSmem {
     double colic[T];                     // and their reciprocals
     double coln[T];                      // populations of the block's columns
     double tk[4][R];                     // per wave: the block's column sums TK_r
-    double tjf[4][R][kWave];             // per lane: sum of A_r over its far rows
-    double tjd[4][R][kWave];             // per lane: sum of A_r over its off-diagonal rows
 };
 
 // Column factors B_r(k) = N_k phi_r(k) of this lane's U columns.
@@ -174,25 +172,31 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
     }
     __syncthreads();
 
-    double Bc[U][R], xk[U];
-    strip_col_factors<KT, U>(a, sm, k0, lane, Bc);
+    double xk[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) xk[u] = sm.xw[lane + kWave * u];
-    // TK_r = sum over the block's columns of B_r(k), kept in this wave's LDS slot
+    {
+        // TK_r = sum over the block's columns of B_r(k), kept in this wave's LDS slot
+        double Bc[U][R];
+        strip_col_factors<KT, U>(a, sm, k0, lane, Bc);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        double s = 0.0;
+        for (int r = 0; r < R; ++r) {
+            double s = 0.0;
 #pragma unroll
-        for (int u = 0; u < U; ++u) s += Bc[u][r];
-        s = wave_sum(s);
-        if (lane == 0) sm.tk[wv][r] = s;
-        sm.tjf[wv][r][lane] = 0.0;
-        sm.tjd[wv][r][lane] = 0.0;
+            for (int u = 0; u < U; ++u) s += Bc[u][r];
+            s = wave_sum(s);
+            if (lane == 0) sm.tk[wv][r] = s;
+        }
     }
-    // far births to k+1 before the 1/(x_{k+1} - x_k) factor: sum_j (v - x_k) rate
-    double accw[U];
+    // Far births to k+1 before the factors B_r(k) / (x_{k+1} - x_k):
+    // acc_r(k) = sum over far rows j of A_r(j) ((x_j + x_k) - x_k).  One
+    // accumulator per (column, r) makes a far pair R FMAs plus the two adds
+    // of its weight, and gives U*R independent FP64 chains.
+    double acc[U][R];
 #pragma unroll
-    for (int u = 0; u < U; ++u) accw[u] = 0.0;
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[u][r] = 0.0;
     double* myrows = sm.rowbuf[wv];
     const BirthSink<WIN> sink{sm.colwin, a.rhs, k0};
     const int rmax = min(n, k0 + T);   // rows j <= last column of the block
@@ -212,7 +216,6 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
     for (int c = c_lo + wv; c < c_hi; c += 4) {
         const int rw0 = c * kStripRows;
         const int rw1 = min(rw0 + kStripRows, rmax);
-        const bool diag = rw0 >= k0;
         // row factors of this chunk: one row per lane
         const int j = rw0 + lane;
         const double xj = j < rw1 ? xj_next : 0.0;
@@ -235,17 +238,9 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
             double dth = 0.0;
 #pragma unroll
             for (int r = 0; r < R; ++r) dth = fma(av[r], sm.tk[wv][r], dth);
-            if (diag) {
-                // rows == columns: bin j loses sum_{k in block} rate_jk,
-                // which also covers the column side of the block's pairs
-                lds_add(&sm.colwin[j - k0], -dth);
-            } else {
-                // issued after the next chunk's loads, so the pair loop below
-                // hides its latency
-                global_add(&a.rhs[j], -dth);
-#pragma unroll
-                for (int r = 0; r < R; ++r) sm.tjd[wv][r][lane] += av[r];
-            }
+            // issued after the next chunk's loads, so the pair loop below
+            // hides its latency
+            global_add(&a.rhs[j], -dth);
         } else {
 #pragma unroll
             for (int r = 0; r < R; ++r) av[r] = 0.0;
@@ -257,7 +252,7 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
 
         const int nrows = rw1 - rw0;
         const double xjmax = __shfl(xj, nrows - 1, kWave);
-        bool ok = !diag;
+        bool ok = true;   // strip rows lie below the band: never on the diagonal
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int k = k0 + lane + kWave * u;
@@ -266,15 +261,8 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
         if (__all(ok)) {
             // every pair lands in (k, k+1), or clips at column n-1 (taken as
             // weight w = 0 there: 1/(x_{k+1} - x_k) is 0 for k = n-1)
-#pragma unroll
-            for (int r = 0; r < R; ++r) sm.tjf[wv][r][lane] += av[r];
-            // Two rows at a time with their own accumulators: 2U independent
-            // FP64 chains (FP64 results take many cycles to come back, so a
-            // wave needs about eight in flight); the r-outer order keeps
-            // consecutive instructions independent.
-            double accw2[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) accw2[u] = 0.0;
+            // Two rows at a time; the second row's update of an accumulator
+            // comes U*R instructions after the first's.
             int jj = 0;
             for (; jj + 1 < nrows; jj += 2) {
                 const double* r0 = myrows + jj * RS;
@@ -286,37 +274,31 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
                     a1[r] = r1[r];
                 }
                 const double x0 = r0[R], x1 = r1[R];
-                double p0[U], p1[U];
+                double w0[U], w1[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    p0[u] = a0[0] * Bc[u][0];
-                    p1[u] = a1[0] * Bc[u][0];
+                    w0[u] = (x0 + xk[u]) - xk[u];
+                    w1[u] = (x1 + xk[u]) - xk[u];
                 }
 #pragma unroll
-                for (int r = 1; r < R; ++r)
+                for (int r = 0; r < R; ++r)
 #pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        p0[u] = fma(a0[r], Bc[u][r], p0[u]);
-                        p1[u] = fma(a1[r], Bc[u][r], p1[u]);
-                    }
+                    for (int u = 0; u < U; ++u) acc[u][r] = fma(a0[r], w0[u], acc[u][r]);
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    accw[u] = fma((x0 + xk[u]) - xk[u], p0[u], accw[u]);
-                    accw2[u] = fma((x1 + xk[u]) - xk[u], p1[u], accw2[u]);
-                }
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int u = 0; u < U; ++u) acc[u][r] = fma(a1[r], w1[u], acc[u][r]);
             }
             if (jj < nrows) {
                 const double* r0 = myrows + jj * RS;
+                const double x0 = r0[R];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    double p = r0[0] * Bc[u][0];
+                    const double w0 = (x0 + xk[u]) - xk[u];
 #pragma unroll
-                    for (int r = 1; r < R; ++r) p = fma(r0[r], Bc[u][r], p);
-                    accw[u] = fma((r0[R] + xk[u]) - xk[u], p, accw[u]);
+                    for (int r = 0; r < R; ++r) acc[u][r] = fma(r0[r], w0, acc[u][r]);
                 }
             }
-#pragma unroll
-            for (int u = 0; u < U; ++u) accw[u] += accw2[u];
         } else {
             // Pairs near the diagonal band: each pair adds its two
             // Kumar-Ramkrishna parts to the LDS window (global atomics
@@ -324,12 +306,15 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
             // v = x_j + x_k for its column; rows run upward, so it only moves
             // up, mostly by at most one bin per row.  One column at a time
             // keeps a single bracket live.
-            auto xat = [&](int i) { return (i - k0 <= WIN) ? sm.xw[i - k0] : a.x[i]; };
+            auto xat = [&](int i) { return (i - k0 <= WIN) ? lds_ld(&sm.xw[i - k0]) : glb_ld(&a.x[i]); };
             auto idat = [&](int i) {
                 const int o = i - k0;
-                return o < T ? sm.idw[o]
-                     : (o < WIN ? 1.0 / (sm.xw[o + 1] - sm.xw[o]) : 1.0 / (a.x[i + 1] - a.x[i]));
+                return o < T ? lds_ld(&sm.idw[o])
+                     : (o < WIN ? 1.0 / (lds_ld(&sm.xw[o + 1]) - lds_ld(&sm.xw[o]))
+                                : 1.0 / (glb_ld(&a.x[i + 1]) - glb_ld(&a.x[i])));
             };
+            double Bc[U][R];
+            strip_col_factors<KT, U>(a, sm, k0, lane, Bc);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int k = k0 + lane + kWave * u;
@@ -337,7 +322,7 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
                 int cur = k;
                 double xc = xk[u], xc1 = k < n - 1 ? sm.xw[lane + kWave * u + 1] : xk[u];
                 double idc = sm.idw[lane + kWave * u];
-                double accc = 0.0;
+                double accc = 0.0, dk = 0.0;
                 const int jend = min(nrows, k - rw0 + 1);   // rows j <= k
                 for (int jj = 0; jj < jend; ++jj) {
                     const double* rp = myrows + jj * RS;
@@ -345,6 +330,7 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
 #pragma unroll
                     for (int r = 1; r < R; ++r) pr = fma(rp[r], Bc[u][r], pr);
                     if (rw0 + jj == k) pr *= 0.5;
+                    dk += pr;
                     const double v = rp[R] + xk[u];
                     if (v >= xlast) {
                         // upper clip (src/aggregation_kernels.cuh:191-192)
@@ -369,34 +355,31 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
                     }
                 }
                 if (accc != 0.0) sink.put(n - 1, accc);
+                if (dk != 0.0) sink.put(k, -dk);   // column k's deaths
             }
         }
         // the next chunk rewrites this wave's rows
         wave_lds_sync();
     }
 
-    // ---- this wave's column totals: far births and off-diagonal deaths ----
-    double TJF[R], TJD[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        TJF[r] = wave_sum(sm.tjf[wv][r][lane]);
-        TJD[r] = wave_sum(sm.tjd[wv][r][lane]);
-    }
+    // ---- far pairs of this wave's chunks: a far pair (j, k) adds
+    // rate (1 - w) to bin k, rate w to bin k+1 and takes rate from bin k, so
+    // bin k nets -rate w and bin k+1 +rate w ----
+    double Bc[U][R];
+    strip_col_factors<KT, U>(a, sm, k0, lane, Bc);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int c = lane + kWave * u;
         const int k = k0 + c;
         if (k < n) {
-            double cs = 0.0, dth = 0.0;
+            double accw = 0.0;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                cs = fma(TJF[r], Bc[u][r], cs);
-                dth = fma(TJD[r], Bc[u][r], dth);
+            for (int r = 0; r < R; ++r) accw = fma(Bc[u][r], acc[u][r], accw);
+            const double hi = accw * sm.idw[c];   // 0 for k = n-1
+            if (hi != 0.0) {
+                lds_add(&sm.colwin[c], -hi);
+                lds_add(&sm.colwin[c + 1], hi);
             }
-            const double hi = accw[u] * sm.idw[c];   // 0 for k = n-1
-            const double lo = (cs - hi) - dth;
-            if (lo != 0.0) lds_add(&sm.colwin[c], lo);
-            if (hi != 0.0) lds_add(&sm.colwin[c + 1], hi);
         }
     }
     __syncthreads();

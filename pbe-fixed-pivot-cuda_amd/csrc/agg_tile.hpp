@@ -324,10 +324,10 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
             // sum rate and sum w*rate of the pairs landing in it, and emits
             // them when the bracket moves on.  Far pairs are the case cur = k.
             const BirthSink<WIN> sink{colwin, a.rhs, k0};
-            auto xat = [&](int i) { return (i - k0 <= WIN) ? sm.xw[i - k0] : a.x[i]; };
+            auto xat = [&](int i) { return (i - k0 <= WIN) ? lds_ld(&sm.xw[i - k0]) : glb_ld(&a.x[i]); };
             auto idat = [&](int i) {
                 const int o = i - k0;
-                return o < WIN ? sm.idw[o] : 1.0 / (a.x[i + 1] - a.x[i]);
+                return o < WIN ? lds_ld(&sm.idw[o]) : 1.0 / (glb_ld(&a.x[i + 1]) - glb_ld(&a.x[i]));
             };
             // Bracket of v = x_j + x_k beyond (k, k+1): a binary search in the
             // LDS copy of the window's pivots, or over all pivots past it.

@@ -81,7 +81,7 @@ extern "C" hipError_t pbe_launch_aggregation_rhs_ex(const double* N, const doubl
     if (u == 0) u = pbe::default_tile_u(p->n, p->kernel_type);
     if (w == 0) w = pbe::default_band_split(p->n, p->kernel_type);
     if (u != 1 && u != 2 && u != 4 && u != 8) return hipErrorInvalidValue;
-    if (w != 1 && w != 2 && w != 4 && w != 8) return hipErrorInvalidValue;
+    if (w != 1 && w != 2 && w != 4 && w != 8 && w != 16) return hipErrorInvalidValue;
     hipError_t e = pbe::launch_aggregation(N, x, rhs, *p, u, w, stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();

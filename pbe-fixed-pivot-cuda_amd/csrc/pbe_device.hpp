@@ -22,6 +22,21 @@ __device__ __forceinline__ void global_add(double* p, double v)
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Loads typed with their address space.  A choice between an LDS element and
+// a global one written as `cond ? lds[i] : glob[j]` becomes a single generic
+// (flat) load, which counts against both the LDS and the vector-memory
+// counters and so waits for every global atomic still in flight; with the
+// address space fixed, the two stay a ds_read and a global_load.
+__device__ __forceinline__ double lds_ld(const double* p)
+{
+    return *(const __attribute__((address_space(3))) double*)p;
+}
+
+__device__ __forceinline__ double glb_ld(const double* p)
+{
+    return *(const __attribute__((address_space(1))) double*)p;
+}
+
 // Move a double across lanes with a DPP control (both 32-bit halves); lanes
 // whose row is not in ROW_MASK read 0.
 template <int CTRL, int ROW_MASK>

@@ -160,10 +160,11 @@ def test_deterministic_argument_errors(pbe):
 
 
 def test_aggregation_option_errors(pbe):
-    """tile_u or band_split outside {0,1,2,4,8} are refused."""
+    """tile_u outside {0,1,2,4,8} or band_split outside {0,1,2,4,8,16} are
+    refused."""
     lib = pbe.library()
     A = pbe.AggregationParams(n=100, kernel_type=5)
-    for u, w in ((3, 0), (16, 0), (4, 3), (4, 16), (4, -1)):
+    for u, w in ((3, 0), (16, 0), (4, 3), (4, 32), (4, -1)):
         opt = pbe.AggOptions(u, w)
         assert lib.pbe_launch_aggregation_rhs_ex(FAKE, FAKE, FAKE, ctypes.byref(A),
                                                  ctypes.byref(opt), None) == INVALID_VALUE

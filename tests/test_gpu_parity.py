@@ -204,7 +204,7 @@ def test_every_tile_shape(pbe, oracle_mod, u, n, kernel):
     drhs = hip.DeviceArray(n)
     drhs.zero()
     p = pbe.agg_params(n, agg)
-    hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=u, band_split=8 // u))
+    hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=u, band_split=16 // u))
     hip.synchronize()
     ref, A, D = _oracle(oracle_mod, N, x, agg=agg)
     got = drhs.download()
@@ -225,3 +225,29 @@ def test_strips_jittered_grid(pbe, oracle_mod):
     ref, A, D = _oracle(oracle_mod, N, x, agg=agg)
     assert_parity(got, ref, A, what="strips jittered n=12000")
     assert_m1(got, x, A, D, what="strips jittered n=12000")
+
+
+@pytest.mark.parametrize("kernel", range(6))
+def test_band_every_split(pbe, oracle_mod, kernel):
+    """n = 9000: the column strips plus the diagonal band, for every band
+    split and the 128-, 256- and 512-column blocks."""
+    from pbe_hip import hip, problems as P
+    n = 9000
+    x = P.geometric_grid(n, 1e-18, 1.02)
+    N = P.lognormal_ic(x, x[n // 3], 1.5, 1e12, noise=0.05, seed=40 + kernel)
+    log_x0, inv_log_r = P.grid_lookup(x)
+    agg = dict(kernel=kernel, beta0=1e-17 if kernel != 2 else 1e-17 / x[n // 3],
+               beta_br=6.73e-18, beta_sh=1e-3, log_x0=log_x0, inv_log_r=inv_log_r)
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg)
+    dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
+    drhs = hip.DeviceArray(n)
+    p = pbe.agg_params(n, agg)
+    for u in (2, 4, 8):
+        for split in (1, 2, 4, 8, 16):
+            drhs.zero()
+            hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=u, band_split=split))
+            hip.synchronize()
+            got = drhs.download()
+            what = f"kernel {kernel} U={u} split {split}"
+            assert_parity(got, ref, A, what=what)
+            assert_m1(got, x, A, D, what=what)
