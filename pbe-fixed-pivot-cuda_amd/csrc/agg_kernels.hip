@@ -96,21 +96,26 @@ void agg_tiles_kernel(AggArgs a, int S)
     }
 }
 
-// Rows below the band, as column strips over the resident grid.  At least 3
-// waves per SIMD (3 workgroups per CU, VGPRs capped at 168), 2 for the
-// widest strips.
+// Rows below the band, as column strips over the resident grid: one
+// workgroup of StripWaves<U> waves per compute unit.
 template <int KT, int U>
-__global__ __launch_bounds__(256, (U >= 8 ? 2 : 3))
+__global__ __launch_bounds__(kWave * StripWaves<U>::value, 1)
 void agg_strip_kernel(AggArgs a, StripPlan plan)
 {
     __shared__ StripSmem<KT, U> sm;
-    const long long C = plan.total, G = gridDim.x, g = blockIdx.x;
+    const long long CT = strip_cost_total(plan), G = gridDim.x, g = blockIdx.x;
     int qa, ca, qb, cb;
-    strip_first_unit(plan, C * g / G, qa, ca);
-    strip_first_unit(plan, C * (g + 1) / G, qb, cb);
-    for (int q = qa, c = ca; q < qb || (q == qb && c < cb); ++q, c = 0) {
+    strip_first_unit(plan, strip_chunk_at_cost(plan, CT * g / G), qa, ca);
+    strip_first_unit(plan, strip_chunk_at_cost(plan, CT * (g + 1) / G), qb, cb);
+    // visits: blocks qa .. qe - 1 (the last one ends at chunk cb of block qb)
+    if (qa > qb || (qa == qb && ca >= cb)) return;   // empty share
+    const int qe = cb > 0 ? qb + 1 : qb;
+    StripCols<KT, U> cols;
+    cols.load(a, qa);
+    for (int i = threadIdx.x; i < StripSmem<KT, U>::WIN; i += blockDim.x) sm.colwin[i] = 0.0;
+    for (int q = qa, c = ca; q < qe; ++q, c = 0) {
         const int hi = (q == qb) ? cb : strip_nc(plan, q);
-        if (c < hi) agg_strip_visit<KT, U>(a, q, c, hi, sm);
+        agg_strip_visit<KT, U>(a, q, c, hi, q + 1 < qe ? q + 1 : -1, cols, sm);
     }
 }
 
@@ -128,7 +133,8 @@ static int resident_groups()
         cus <= 0)
         cus = 256;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(&agg_strip_kernel<KT, U>), 256, 0) !=
+            &per_cu, reinterpret_cast<const void*>(&agg_strip_kernel<KT, U>),
+            kWave * StripWaves<U>::value, 0) !=
             hipSuccess || per_cu <= 0)
         per_cu = 1;
     v = cus * per_cu;
@@ -147,7 +153,7 @@ static hipError_t launch_agg_t(AggArgs args, int split, hipStream_t stream)
     plan.nb = args.nb;
     plan.total = strip_total(plan);
     const long long G = resident_groups<KT, U>();
-    if (plan.total < G) {
+    if (plan.total < G * StripWaves<U>::value / 4) {
         // too little far work for the strips: one launch of all tiles
         const long long tiles = (long long)args.nb * (args.nb + 1) / 2;
         const long long items = band_items + tiles - band_tiles(args.nb);
@@ -165,8 +171,8 @@ static hipError_t launch_agg_t(AggArgs args, int split, hipStream_t stream)
         hipLaunchKernelGGL((agg_band_kernel<KT, U>), dim3((unsigned)band_items), dim3(256), 0,
                            stream, args, split);
     }
-    hipLaunchKernelGGL((agg_strip_kernel<KT, U>), dim3((unsigned)G), dim3(256), 0, stream,
-                       args, plan);
+    hipLaunchKernelGGL((agg_strip_kernel<KT, U>), dim3((unsigned)G),
+                       dim3(kWave * StripWaves<U>::value), 0, stream, args, plan);
     return hipSuccess;
 }
 

@@ -14,9 +14,10 @@
 // (pivots, cube roots, the LDS window) and the flush of the window are paid
 // once per visit of a block rather than once per tile.
 //
-// Within a visit the four waves take the visit's chunks round robin; all
-// waves hold all T columns (U per lane) and a chunk's 64 row factors are
-// staged in the wave's own LDS rows.  A chunk that is not entirely far (a
+// One workgroup per compute unit, of StripWaves<U> waves (3 per SIMD).  A
+// visit's rows are split evenly over the waves; all waves hold all T columns
+// (U per lane) and take their rows 64 at a time, staging the row factors in
+// the wave's own LDS rows.  A chunk that is not entirely far (a
 // grid finer than the band) takes a per-pair path with LDS atomics.
 #pragma once
 
@@ -27,6 +28,13 @@
 namespace pbe {
 
 constexpr int kStripRows = 64;       // rows per chunk
+
+// Waves per strip workgroup: three per SIMD; two for the widest blocks, whose
+// waves need more than 168 VGPRs.
+template <int U>
+struct StripWaves {
+    static constexpr int value = U >= 8 ? 8 : 12;
+};
 constexpr int kStripExtra = 256;     // near-birth reach of the LDS window beyond the block
 
 // Chunk layout of one launch: block q >= 2 has (q - 1) U chunks.
@@ -52,6 +60,40 @@ __host__ __device__ __forceinline__ long long strip_prefix(const StripPlan& s, i
 __host__ __device__ __forceinline__ long long strip_total(const StripPlan& s)
 {
     return strip_prefix(s, s.nb);
+}
+
+// Workgroup shares.  A visit's column set-up and window flush, during which
+// the workgroup's waves wait for one another, cost about five chunks of pairs,
+// so the shares split the cost line evenly: chunk t of block
+// q sits at cost t + kVisitCost (q - 1), block q's visit charge lying just
+// before its chunk 0.  A share that crosses into a new block carries that
+// block's charge and so holds one chunk fewer.
+constexpr int kVisitCost = 5;
+
+__host__ __device__ __forceinline__ long long strip_cost_start(const StripPlan& s, int q)
+{
+    return strip_prefix(s, q) + (long long)kVisitCost * (q - 2);
+}
+
+__host__ __device__ __forceinline__ long long strip_cost_total(const StripPlan& s)
+{
+    return strip_cost_start(s, s.nb);
+}
+
+// First chunk at or after cost position f (s.total past the end).
+__device__ __forceinline__ long long strip_chunk_at_cost(const StripPlan& s, long long f)
+{
+    if (f >= strip_cost_total(s)) return s.total;
+    // q = m + 2 with U m (m + 1) / 2 + kVisitCost m <= f
+    const double b = 0.5 * s.U + kVisitCost;
+    int m = (int)floor((sqrt(b * b + 2.0 * s.U * (double)f) - b) / s.U);
+    if (m < 0) m = 0;
+    while (m > 0 && strip_cost_start(s, m + 2) > f) --m;
+    while (m + 3 <= s.nb && strip_cost_start(s, m + 3) <= f) ++m;
+    const int q = m + 2;
+    long long off = f - strip_cost_start(s, q) - kVisitCost;
+    if (off < 0) off = 0;
+    return strip_prefix(s, q) + off;
 }
 
 // Chunk (q, c) at linear index t; (nb, 0) past the end.
@@ -86,14 +128,14 @@ struct StripSmem {
     static constexpr int RS = (R + 2) & ~1;
     static constexpr int T = kWave * U;
     static constexpr int WIN = T + kStripExtra;
-    double rowbuf[4][kStripRows * RS];   // per wave: row factors A_r(j) and x_j
+    static constexpr int NWV = StripWaves<U>::value;
+    double rowbuf[NWV][kStripRows * RS]; // per wave: row factors A_r(j) and x_j
     double colwin[WIN];                  // births/deaths of the window bins
     double xw[WIN + 1];                  // pivots of the window bins
     double idw[T];                       // 1/(x_{i+1} - x_i) of the block's columns
     double colc[T];                      // cube roots of the block's pivots
     double colic[T];                     // and their reciprocals
     double coln[T];                      // populations of the block's columns
-    double tk[4][R];                     // per wave: the block's column sums TK_r
 };
 
 // Column factors B_r(k) = N_k phi_r(k) of this lane's U columns.
@@ -120,52 +162,73 @@ __device__ __forceinline__ void strip_col_factors(const AggArgs& a, const StripS
     }
 }
 
-// One visit of column block q by the workgroup: chunks [c_lo, c_hi).
+// Column data of block q held in registers between visits: window pivots,
+// populations and right neighbours of the block's columns.
+template <int KT, int U>
+struct StripCols {
+    static constexpr int T = StripSmem<KT, U>::T;
+    static constexpr int WIN = StripSmem<KT, U>::WIN;
+    static constexpr int NWV = StripSmem<KT, U>::NWV;
+    static constexpr int NTH = kWave * NWV;   // threads of the workgroup
+    static constexpr int NW = (WIN + 1 + NTH - 1) / NTH;
+    static constexpr int NT = (T + NTH - 1) / NTH;
+    double xv[NW], nv[NT], xn[NT];
+
+    __device__ __forceinline__ void load(const AggArgs& a, int q)
+    {
+        const int tid = threadIdx.x, n = a.n, k0 = q * T;
+#pragma unroll
+        for (int t = 0; t < NW; ++t) {
+            const int i = tid + NTH * t, b = k0 + i;
+            xv[t] = (i <= WIN && b < n) ? a.x[b] : __builtin_inf();
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int i = tid + NTH * t, k = k0 + i;
+            nv[t] = (i < T && k < n) ? a.N[k] : 0.0;
+            xn[t] = (i < T && k + 1 < n) ? a.x[k + 1] : __builtin_inf();
+        }
+    }
+};
+
+// One visit of column block q by the workgroup: chunks [c_lo, c_hi).  On
+// entry `cols` holds block q's column data and the LDS window is zero; on exit
+// `cols` holds block next_q's (if next_q >= 0, loaded before the window's
+// global atomics so those loads do not wait behind them) and this thread's
+// window bins are flushed and zero again.
 template <int KT, int U>
 __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, const int c_lo,
-                                                const int c_hi, StripSmem<KT, U>& sm)
+                                                const int c_hi, const int next_q,
+                                                StripCols<KT, U>& cols, StripSmem<KT, U>& sm)
 {
     using K = Kern<KT>;
+    using Cols = StripCols<KT, U>;
     constexpr int R = K::R;
     constexpr int RS = StripSmem<KT, U>::RS;
     constexpr int T = StripSmem<KT, U>::T;
     constexpr int WIN = StripSmem<KT, U>::WIN;
+    constexpr int NTH = Cols::NTH;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
     const int n = a.n;
     const int k0 = q * T;
     const double xlast = a.x[n - 1];
 
-    // ---- column set-up, once per visit: all global loads first ----
-    constexpr int NW = (WIN + 1 + 255) / 256;
-    constexpr int NT = (T + 255) / 256;
-    double xv[NW], nv[NT], xn[NT];
+    // ---- column set-up, once per visit ----
 #pragma unroll
-    for (int t = 0; t < NW; ++t) {
-        const int i = tid + 256 * t, b = k0 + i;
-        xv[t] = (i <= WIN && b < n) ? a.x[b] : __builtin_inf();
-    }
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int i = tid + 256 * t, k = k0 + i;
-        nv[t] = (i < T && k < n) ? a.N[k] : 0.0;
-        xn[t] = (i < T && k + 1 < n) ? a.x[k + 1] : __builtin_inf();
-    }
-    for (int i = tid; i < WIN; i += 256) sm.colwin[i] = 0.0;
-#pragma unroll
-    for (int t = 0; t < NW; ++t) {
-        const int i = tid + 256 * t;
-        if (i <= WIN) sm.xw[i] = xv[t];
+    for (int t = 0; t < Cols::NW; ++t) {
+        const int i = tid + NTH * t;
+        if (i <= WIN) sm.xw[i] = cols.xv[t];
     }
     // the block's columns: 1/(x_{k+1} - x_k) (the near path forms the rest of
     // the window's when it needs them), populations, cube roots
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int i = tid + 256 * t, k = k0 + i;
+    for (int t = 0; t < Cols::NT; ++t) {
+        const int i = tid + NTH * t, k = k0 + i;
         if (i < T) {
-            sm.idw[i] = k < n - 1 ? 1.0 / (xn[t] - xv[t]) : 0.0;
-            sm.coln[i] = nv[t];
+            sm.idw[i] = k < n - 1 ? 1.0 / (cols.xn[t] - cols.xv[t]) : 0.0;
+            sm.coln[i] = cols.nv[t];
             double cb = 0.0, icb = 0.0;
-            if (k < n) pivot_roots<KT>(xv[t], cb, icb);
+            if (k < n) pivot_roots<KT>(cols.xv[t], cb, icb);
             sm.colc[i] = cb;
             sm.colic[i] = icb;
         }
@@ -175,8 +238,9 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
     double xk[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) xk[u] = sm.xw[lane + kWave * u];
+    // TK_r = sum over the block's columns of B_r(k) (the same in every lane)
+    double TK[R];
     {
-        // TK_r = sum over the block's columns of B_r(k), kept in this wave's LDS slot
         double Bc[U][R];
         strip_col_factors<KT, U>(a, sm, k0, lane, Bc);
 #pragma unroll
@@ -184,8 +248,7 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
             double s = 0.0;
 #pragma unroll
             for (int u = 0; u < U; ++u) s += Bc[u][r];
-            s = wave_sum(s);
-            if (lane == 0) sm.tk[wv][r] = s;
+            TK[r] = wave_sum(s);
         }
     }
     // Far births to k+1 before the factors B_r(k) / (x_{k+1} - x_k):
@@ -202,31 +265,36 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
     const int rmax = min(n, k0 + T);   // rows j <= last column of the block
     wave_lds_sync();
 
-    // This lane's row of the next chunk is loaded one chunk ahead, before the
-    // current chunk issues its global atomic: a load issued after the atomic
-    // would wait for it (vector memory counts retire in order).
-    double xj_next = 0.0, Nj_next = 0.0;
-    {
-        const int j = (c_lo + wv) * kStripRows + lane;
-        if (c_lo + wv < c_hi && j < rmax) {
-            xj_next = a.x[j];
-            Nj_next = a.N[j];
+    // The visit's rows are split evenly over the waves; a wave takes its rows
+    // in pieces of up to 64 (one row per lane).  Its issue priority falls
+    // from 3 to 0 as it moves through them: the SIMD otherwise favours its
+    // oldest wave, which would finish first and leave the others to run at
+    // lower occupancy.  This lane's row of the next piece is loaded one piece
+    // ahead, before the current piece issues its global atomic: a load
+    // issued after the atomic would wait for it (vector memory counts retire
+    // in order).
+    const int vr0 = c_lo * kStripRows, vr1 = min(c_hi * kStripRows, rmax);
+    const int w_lo = vr0 + (int)((long long)(vr1 - vr0) * wv / Cols::NWV);
+    const int w_hi = vr0 + (int)((long long)(vr1 - vr0) * (wv + 1) / Cols::NWV);
+    auto load_row = [&](int p, double& xr, double& nr) {
+        const int j = p + lane;
+        xr = 0.0;
+        nr = 0.0;
+        if (j < min(p + kStripRows, w_hi)) {
+            xr = a.x[j];
+            nr = a.N[j];
         }
-    }
-    for (int c = c_lo + wv; c < c_hi; c += 4) {
-        const int rw0 = c * kStripRows;
-        const int rw1 = min(rw0 + kStripRows, rmax);
-        // row factors of this chunk: one row per lane
+    };
+    double xj_next, Nj_next;
+    load_row(w_lo, xj_next, Nj_next);
+    for (int rw0 = w_lo; rw0 < w_hi; rw0 += kStripRows) {
+        set_wave_priority(3 - 4 * (rw0 - w_lo) / (w_hi - w_lo));
+        const int rw1 = min(rw0 + kStripRows, w_hi);
+        // row factors of this piece: one row per lane
         const int j = rw0 + lane;
         const double xj = j < rw1 ? xj_next : 0.0;
         const double Nj = Nj_next;
-        {
-            const int jn = j + 4 * kStripRows;
-            if (c + 4 < c_hi && jn < rmax) {
-                xj_next = a.x[jn];
-                Nj_next = a.N[jn];
-            }
-        }
+        load_row(rw0 + kStripRows, xj_next, Nj_next);
         double av[R];
         if (j < rw1) {
             double cb, icb;
@@ -237,7 +305,7 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
             // death of row j against the block's columns (separable)
             double dth = 0.0;
 #pragma unroll
-            for (int r = 0; r < R; ++r) dth = fma(av[r], sm.tk[wv][r], dth);
+            for (int r = 0; r < R; ++r) dth = fma(av[r], TK[r], dth);
             // issued after the next chunk's loads, so the pair loop below
             // hides its latency
             global_add(&a.rhs[j], -dth);
@@ -358,7 +426,7 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
                 if (dk != 0.0) sink.put(k, -dk);   // column k's deaths
             }
         }
-        // the next chunk rewrites this wave's rows
+        // the next piece rewrites this wave's rows
         wave_lds_sync();
     }
 
@@ -384,13 +452,14 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
     }
     __syncthreads();
 
-    // ---- flush the window ----
-    for (int i = tid; i < WIN; i += 256) {
+    // ---- the next block's column loads, then flush the window ----
+    if (next_q >= 0) cols.load(a, next_q);
+    for (int i = tid; i < WIN; i += NTH) {
         const int k = k0 + i;
         const double v = sm.colwin[i];
+        sm.colwin[i] = 0.0;
         if (k < n && v != 0.0) global_add(&a.rhs[k], v);
     }
-    __syncthreads();
 }
 
 }  // namespace pbe

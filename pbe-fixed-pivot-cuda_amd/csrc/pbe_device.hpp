@@ -37,6 +37,18 @@ __device__ __forceinline__ double glb_ld(const double* p)
     return *(const __attribute__((address_space(1))) double*)p;
 }
 
+// Issue priority of this wave among the waves of its SIMD (0 lowest, 3
+// highest); p must be the same in every lane.
+__device__ __forceinline__ void set_wave_priority(int p)
+{
+    switch (p) {
+    case 3: __builtin_amdgcn_s_setprio(3); break;
+    case 2: __builtin_amdgcn_s_setprio(2); break;
+    case 1: __builtin_amdgcn_s_setprio(1); break;
+    default: __builtin_amdgcn_s_setprio(0); break;
+    }
+}
+
 // Move a double across lanes with a DPP control (both 32-bit halves); lanes
 // whose row is not in ROW_MASK read 0.
 template <int CTRL, int ROW_MASK>
