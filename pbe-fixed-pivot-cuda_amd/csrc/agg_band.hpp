@@ -31,6 +31,16 @@ constexpr int kBandRows = 2 * kBandT;     // rows of the band: block below + blo
 constexpr int kBandWin = kBandT + 256;    // target bins held in LDS
 constexpr int kBandStreams = 4;
 
+// Offset segments of the band: segment s of a block covers offsets
+// [e[s], e[s + 1]) (block 0, whose columns have fewer rows below them, uses
+// e0).  The host picks them so that segments cost about the same
+// (band_segments in agg_kernels.hip).
+constexpr int kBandMaxSeg = 16;
+struct BandSegs {
+    short e[kBandMaxSeg + 1];
+    short e0[kBandMaxSeg + 1];
+};
+
 template <int KT>
 struct BandSmem {
     static constexpr int R = Kern<KT>::R;
@@ -41,11 +51,10 @@ struct BandSmem {
     double red[4][2 * R];          // per-wave TK_r (block columns), TJ_r (rows below the block)
 };
 
-// One (block, offset segment) item.  nseg segments split the offsets
-// [0, 2T) evenly; every thread calls it.
+// One (block, offset segment) item; every thread calls it.
 template <int KT>
 __device__ __forceinline__ void agg_band_sweep(const AggArgs& a, const int q, const int seg,
-                                               const int nseg, BandSmem<KT>& sm)
+                                               const BandSegs& segs, BandSmem<KT>& sm)
 {
     using K = Kern<KT>;
     constexpr int R = K::R;
@@ -184,7 +193,8 @@ __device__ __forceinline__ void agg_band_sweep(const AggArgs& a, const int q, co
 
     // ---- births: the segment's offsets as four interleaved streams ----
     const int emax = k - jlo;                       // largest offset of this column
-    const int e0 = seg * kBandRows / nseg, e1 = (seg + 1) * kBandRows / nseg;
+    const int e0 = q == 0 ? segs.e0[seg] : segs.e[seg];
+    const int e1 = q == 0 ? segs.e0[seg + 1] : segs.e[seg + 1];
     const int L = (e1 - e0 + NS - 1) / NS;          // steps per stream
     // Each stream's bracket at its first pair, all four searches together:
     // a stream's first pair is where its bracket is furthest from (k, k+1),

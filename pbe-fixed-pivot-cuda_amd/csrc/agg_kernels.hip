@@ -39,6 +39,7 @@
 //  * One flush per tile or visit of a block: the LDS window is added to rhs
 //    with FP64 global atomics (zero entries skipped).
 #include <atomic>
+#include <cmath>
 
 #include "agg_band.hpp"
 #include "agg_strip.hpp"
@@ -70,10 +71,53 @@ void agg_band_kernel(AggArgs a, int S)
 // item = (block, offset segment).
 template <int KT>
 __global__ __launch_bounds__(256, 3)
-void agg_sweep_kernel(AggArgs a, int nseg)
+void agg_sweep_kernel(AggArgs a, int nseg, BandSegs segs)
 {
     __shared__ BandSmem<KT> sm;
-    agg_band_sweep<KT>(a, blockIdx.x / nseg, blockIdx.x % nseg, nseg, sm);
+    agg_band_sweep<KT>(a, blockIdx.x / nseg, blockIdx.x % nseg, segs, sm);
+}
+
+// Offset segments of equal estimated cost.  A wave walks every offset of its
+// segment whatever share of its lanes holds a pair there, so a step (one
+// offset) costs 1; a move of the bracket costs about 2 more.  In block 0 no
+// column has a pair beyond offset T.  On a geometric grid of ratio r the
+// bracket of x_{k-e} + x_k sits log_r(1 + r^-e) bins above k, so offsets
+// [0, e) make log_r 2 - log_r(1 + r^-e) moves.  Without a ratio (a general
+// grid) the segments are equal.
+static BandSegs band_segments(int S, double inv_log_r)
+{
+    constexpr int T = kBandT, E = 2 * kBandT;
+    constexpr double kMoveCost = 2.0;
+    BandSegs sg;
+    auto cost = [&](int e, bool corner) {
+        const double fe = corner && e > T ? T : e;
+        double c = fe;
+        if (inv_log_r > 0.0) {
+            const double lr = 1.0 / inv_log_r;
+            c += kMoveCost * (std::log(2.0) - std::log1p(std::exp(-fe * lr))) * inv_log_r;
+        }
+        return c;
+    };
+    for (int corner = 0; corner < 2; ++corner) {
+        short* out = corner ? sg.e0 : sg.e;
+        const double total = cost(E, corner);
+        const int EE = corner ? T : E;
+        for (int s = 0; s <= S; ++s) {
+            int e = s * EE / S;
+            if (inv_log_r > 0.0 && s > 0 && s < S) {
+                const double target = total * s / S;
+                int lo = 0, hi = EE;   // smallest e with cost(e) >= target
+                while (lo < hi) {
+                    const int mid = (lo + hi) / 2;
+                    if (cost(mid, corner) >= target) hi = mid; else lo = mid + 1;
+                }
+                e = lo;
+            }
+            out[s] = (short)e;
+        }
+        for (int s = S + 1; s <= kBandMaxSeg; ++s) out[s] = (short)E;
+    }
+    return sg;
 }
 
 // Small problems (too few far chunks to fill the device once): one launch of
@@ -166,7 +210,7 @@ static hipError_t launch_agg_t(AggArgs args, int split, hipStream_t stream)
     if (T == kBandT) {
         // band as diagonal sweeps: split = offset segments per block
         hipLaunchKernelGGL((agg_sweep_kernel<KT>), dim3((unsigned)(args.nb * split)), dim3(256),
-                           0, stream, args, split);
+                           0, stream, args, split, band_segments(split, args.inv_log_r));
     } else {
         hipLaunchKernelGGL((agg_band_kernel<KT, U>), dim3((unsigned)band_items), dim3(256), 0,
                            stream, args, split);

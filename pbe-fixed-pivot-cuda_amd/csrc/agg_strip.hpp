@@ -14,7 +14,7 @@
 // (pivots, cube roots, the LDS window) and the flush of the window are paid
 // once per visit of a block rather than once per tile.
 //
-// One workgroup per compute unit, of StripWaves<U> waves (3 per SIMD).  A
+// One workgroup per compute unit, of StripWaves<U> waves (2 per SIMD).  A
 // visit's rows are split evenly over the waves; all waves hold all T columns
 // (U per lane) and take their rows 64 at a time, staging the row factors in
 // the wave's own LDS rows.  A chunk that is not entirely far (a
@@ -29,11 +29,11 @@ namespace pbe {
 
 constexpr int kStripRows = 64;       // rows per chunk
 
-// Waves per strip workgroup: three per SIMD; two for the widest blocks, whose
-// waves need more than 168 VGPRs.
+// Waves per strip workgroup: two per SIMD.  (Three per SIMD, at most 168
+// VGPRs each, measured a few percent slower.)
 template <int U>
 struct StripWaves {
-    static constexpr int value = U >= 8 ? 8 : 12;
+    static constexpr int value = 8;
 };
 constexpr int kStripExtra = 256;     // near-birth reach of the LDS window beyond the block
 
