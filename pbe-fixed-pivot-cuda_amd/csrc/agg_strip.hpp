@@ -129,7 +129,7 @@ struct StripSmem {
     static constexpr int T = kWave * U;
     static constexpr int WIN = T + kStripExtra;
     static constexpr int NWV = StripWaves<U>::value;
-    double rowbuf[NWV][kStripRows * RS]; // per wave: row factors A_r(j) and x_j
+    double rowbuf[NWV][(kStripRows + 2) * RS];   // per wave: row factors A_r(j) and x_j
     double colwin[WIN];                  // births/deaths of the window bins
     double xw[WIN + 1];                  // pivots of the window bins
     double idw[T];                       // 1/(x_{i+1} - x_i) of the block's columns
@@ -329,33 +329,50 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
         if (__all(ok)) {
             // every pair lands in (k, k+1), or clips at column n-1 (taken as
             // weight w = 0 there: 1/(x_{k+1} - x_k) is 0 for k = n-1)
-            // Two rows at a time; the second row's update of an accumulator
-            // comes U*R instructions after the first's.
-            int jj = 0;
-            for (; jj + 1 < nrows; jj += 2) {
-                const double* r0 = myrows + jj * RS;
-                const double* r1 = r0 + RS;
-                double a0[R], a1[R];
+            // Rows in pairs, the second row's update of an accumulator U*R
+            // instructions after the first's.  The LDS reads of the next
+            // pair are issued before the FMAs of this one, alternating two
+            // register sets, so their latency hides behind the FMAs (past
+            // the last row they read the wave's two spare rows).
+            auto ld2 = [&](int jr, double (&p0)[R + 1], double (&p1)[R + 1]) {
+                const double* r0 = myrows + jr * RS;
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    a0[r] = r0[r];
-                    a1[r] = r1[r];
+                for (int r = 0; r <= R; ++r) {
+                    p0[r] = r0[r];
+                    p1[r] = r0[RS + r];
                 }
-                const double x0 = r0[R], x1 = r1[R];
+            };
+            auto fma2 = [&](const double (&p0)[R + 1], const double (&p1)[R + 1]) {
                 double w0[U], w1[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    w0[u] = (x0 + xk[u]) - xk[u];
-                    w1[u] = (x1 + xk[u]) - xk[u];
+                    w0[u] = (p0[R] + xk[u]) - xk[u];
+                    w1[u] = (p1[R] + xk[u]) - xk[u];
                 }
 #pragma unroll
                 for (int r = 0; r < R; ++r)
 #pragma unroll
-                    for (int u = 0; u < U; ++u) acc[u][r] = fma(a0[r], w0[u], acc[u][r]);
+                    for (int u = 0; u < U; ++u) acc[u][r] = fma(p0[r], w0[u], acc[u][r]);
 #pragma unroll
                 for (int r = 0; r < R; ++r)
 #pragma unroll
-                    for (int u = 0; u < U; ++u) acc[u][r] = fma(a1[r], w1[u], acc[u][r]);
+                    for (int u = 0; u < U; ++u) acc[u][r] = fma(p1[r], w1[u], acc[u][r]);
+            };
+            double pa0[R + 1], pa1[R + 1], pb0[R + 1], pb1[R + 1];
+            ld2(0, pa0, pa1);
+            int jj = 0;
+            for (; jj + 3 < nrows; jj += 4) {
+                ld2(jj + 2, pb0, pb1);
+                __builtin_amdgcn_sched_barrier(0);
+                fma2(pa0, pa1);
+                ld2(jj + 4, pa0, pa1);
+                __builtin_amdgcn_sched_barrier(0);
+                fma2(pb0, pb1);
+            }
+            // 0 to 3 rows left; rows jj and jj + 1 are in pa
+            if (jj + 1 < nrows) {
+                fma2(pa0, pa1);
+                jj += 2;
             }
             if (jj < nrows) {
                 const double* r0 = myrows + jj * RS;
