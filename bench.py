@@ -7,9 +7,10 @@ symmetric-binary daughters) at n = 16384 on a geometric grid r = 1.02, with a
 seeded lognormal N already resident in HBM.  Work units per step:
 n(n+1)/2 pairs + n*n_quad breakage updates (dense: pairs with N = 0 count).
 
-Prints one JSON line.  ``roofline`` refers to the dominant kernel (the
-aggregation tiles): achieved = 21 FP64 FLOP per pair x pairs / average
-aggregation-kernel duration, measured with HIP events on the launch stream.
+Prints one JSON line.  ``roofline`` refers to the dominant work, the
+aggregation call (at n = 16384 two launches: the diagonal-band sweep and the
+column strips): achieved = 21 FP64 FLOP per pair x pairs / average duration
+of the call, measured with HIP events on the launch stream around it.
 ``cpu_baseline`` times the OpenMP FP64 restatement of the reference
 (oracle/pbe_cpu_port.c) on this host on a bounded sample of the same work.
 """
@@ -26,7 +27,7 @@ sys.path.insert(0, os.path.join(REPO, "pbe-fixed-pivot-cuda_amd", "python"))
 
 FP64_PEAK_TFLOPS = 78.6
 METRIC = "FP64 pair-updates/s, agg+breakage RHS at n=16384; batched & adjoint vs plain"
-TRAFFIC_FILE = os.path.join(REPO, "profiles", "agg_tile_brshear_n16384_traffic.json")
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "agg_brshear_n16384_traffic.json")
 
 
 def parse():
@@ -303,7 +304,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": traffic,
-            "kernel": "agg_tile_kernel<BrownianShear>",
+            "kernel": "aggregation call: agg_sweep_kernel + agg_strip_kernel <BrownianShear>",
             "flop_per_launch": flops,
             "avg_launch_ms": agg_ms,
         },

@@ -3,8 +3,9 @@
     python pbe-fixed-pivot-cuda_amd/autotune.py --out tile_table.inc [--json results.json]
 
 For each size bucket and collision kernel, every block width (64*U columns,
-U = 1, 2, 4, 8 columns per lane) and band split S (row slices per
-diagonal-band tile, S = 1, 2, 4, 8) is timed with HIP events over
+U = 1, 2, 4, 8 columns per lane) and band split S = 1, 2, 4, 8, 16 (row
+slices per diagonal-band tile; offset segments per block for the band
+sweeps of 256-column blocks) is timed with HIP events over
 back-to-back launches on the flocculation workload (lognormal N, geometric
 grid r = 1.02), and the fastest (U, S) is written as a rule {n_max, kernel,
 U, S} in the C include that the library compiles in (csrc/tile_table.inc).  The table is read-only
@@ -21,7 +22,7 @@ sys.path.insert(0, os.path.join(HERE, "python"))
 SIZES = [256, 512, 1024, 2048, 4096, 8192, 16384]
 KERNELS = [0, 1, 2, 3, 4, 5]
 TILES = [1, 2, 4, 8]
-WEIGHTS = [1, 2, 4, 8]
+WEIGHTS = [1, 2, 4, 8, 16]
 
 
 def time_launch(pbe, hip, N, x, agg, u, w, reps):
