@@ -70,7 +70,7 @@ void agg_band_kernel(AggArgs a, int S)
 // Diagonal band of 256-column blocks as diagonal sweeps (agg_band.hpp):
 // item = (block, offset segment).
 template <int KT>
-__global__ __launch_bounds__(256, 3)
+__global__ __launch_bounds__(256, 4)
 void agg_sweep_kernel(AggArgs a, int nseg, BandSegs segs)
 {
     __shared__ BandSmem<KT> sm;

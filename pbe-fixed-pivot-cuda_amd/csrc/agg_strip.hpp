@@ -28,6 +28,7 @@
 namespace pbe {
 
 constexpr int kStripRows = 64;       // rows per chunk
+constexpr int kStripPieceRows = 128; // rows a wave stages and sweeps at a time
 
 // Waves per strip workgroup: two per SIMD.  (Three per SIMD, at most 168
 // VGPRs each, measured a few percent slower.)
@@ -129,13 +130,14 @@ struct StripSmem {
     static constexpr int T = kWave * U;
     static constexpr int WIN = T + kStripExtra;
     static constexpr int NWV = StripWaves<U>::value;
-    double rowbuf[NWV][(kStripRows + 2) * RS];   // per wave: row factors A_r(j) and x_j
+    double rowbuf[NWV][(kStripPieceRows + 2) * RS];   // per wave: row factors A_r(j) and x_j
     double colwin[WIN];                  // births/deaths of the window bins
     double xw[WIN + 1];                  // pivots of the window bins
     double idw[T];                       // 1/(x_{i+1} - x_i) of the block's columns
     double colc[T];                      // cube roots of the block's pivots
     double colic[T];                     // and their reciprocals
     double coln[T];                      // populations of the block's columns
+    int prog[NWV];                       // per wave: rows of the visit done so far
 };
 
 // Column factors B_r(k) = N_k phi_r(k) of this lane's U columns.
@@ -172,22 +174,38 @@ struct StripCols {
     static constexpr int NTH = kWave * NWV;   // threads of the workgroup
     static constexpr int NW = (WIN + 1 + NTH - 1) / NTH;
     static constexpr int NT = (T + NTH - 1) / NTH;
-    double xv[NW], nv[NT], xn[NT];
+    double xv[NW], nv[NT], xn[NT];   // as loaded: apply valid_* before use
 
+    // Unconditional loads, indices clamped into the arrays: a load under a
+    // branch makes the compiler wait for it (and for every vector-memory
+    // operation before it, the window's flush atomics included) at once.
     __device__ __forceinline__ void load(const AggArgs& a, int q)
     {
         const int tid = threadIdx.x, n = a.n, k0 = q * T;
 #pragma unroll
-        for (int t = 0; t < NW; ++t) {
-            const int i = tid + NTH * t, b = k0 + i;
-            xv[t] = (i <= WIN && b < n) ? a.x[b] : __builtin_inf();
-        }
+        for (int t = 0; t < NW; ++t) xv[t] = a.x[min(k0 + tid + NTH * t, n - 1)];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const int i = tid + NTH * t, k = k0 + i;
-            nv[t] = (i < T && k < n) ? a.N[k] : 0.0;
-            xn[t] = (i < T && k + 1 < n) ? a.x[k + 1] : __builtin_inf();
+            const int k = k0 + tid + NTH * t;
+            nv[t] = a.N[min(k, n - 1)];
+            xn[t] = a.x[min(k + 1, n - 1)];
         }
+    }
+    // window pivot t of this thread (inf past the window or the grid)
+    __device__ __forceinline__ double pivot(int n, int q, int t) const
+    {
+        const int i = threadIdx.x + NTH * t, b = q * T + i;
+        return (i <= WIN && b < n) ? xv[t] : __builtin_inf();
+    }
+    __device__ __forceinline__ double pop(int n, int q, int t) const
+    {
+        const int i = threadIdx.x + NTH * t, k = q * T + i;
+        return (i < T && k < n) ? nv[t] : 0.0;
+    }
+    __device__ __forceinline__ double next_pivot(int n, int q, int t) const
+    {
+        const int i = threadIdx.x + NTH * t, k = q * T + i;
+        return (i < T && k + 1 < n) ? xn[t] : __builtin_inf();
     }
 };
 
@@ -214,10 +232,11 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
     const double xlast = a.x[n - 1];
 
     // ---- column set-up, once per visit ----
+    if (tid < Cols::NWV) sm.prog[tid] = 0;
 #pragma unroll
     for (int t = 0; t < Cols::NW; ++t) {
         const int i = tid + NTH * t;
-        if (i <= WIN) sm.xw[i] = cols.xv[t];
+        if (i <= WIN) sm.xw[i] = cols.pivot(n, q, t);
     }
     // the block's columns: 1/(x_{k+1} - x_k) (the near path forms the rest of
     // the window's when it needs them), populations, cube roots
@@ -225,10 +244,11 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
     for (int t = 0; t < Cols::NT; ++t) {
         const int i = tid + NTH * t, k = k0 + i;
         if (i < T) {
-            sm.idw[i] = k < n - 1 ? 1.0 / (cols.xn[t] - cols.xv[t]) : 0.0;
-            sm.coln[i] = cols.nv[t];
+            const double xk_ = cols.pivot(n, q, t);
+            sm.idw[i] = k < n - 1 ? 1.0 / (cols.next_pivot(n, q, t) - xk_) : 0.0;
+            sm.coln[i] = cols.pop(n, q, t);
             double cb = 0.0, icb = 0.0;
-            if (k < n) pivot_roots<KT>(cols.xv[t], cb, icb);
+            if (k < n) pivot_roots<KT>(xk_, cb, icb);
             sm.colc[i] = cb;
             sm.colic[i] = icb;
         }
@@ -266,60 +286,91 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
     wave_lds_sync();
 
     // The visit's rows are split evenly over the waves; a wave takes its rows
-    // in pieces of up to 64 (one row per lane).  Its issue priority falls
-    // from 3 to 0 as it moves through them: the SIMD otherwise favours its
-    // oldest wave, which would finish first and leave the others to run at
-    // lower occupancy.  This lane's row of the next piece is loaded one piece
-    // ahead, before the current piece issues its global atomic: a load
-    // issued after the atomic would wait for it (vector memory counts retire
+    // in pieces of up to 128, two rows per lane, whose factor computations
+    // (cube-root chains) run interleaved.  Every 32 rows a wave posts its
+    // progress in LDS and sets its issue priority from its lead over the
+    // workgroup's mean (ahead: low, behind: high): the SIMD otherwise favours
+    // its oldest wave, which would finish first while the younger one on the
+    // same SIMD lags (measured: 20 us against 30 us) and the workgroup waits
+    // for it at the end of the visit.  This lane's rows of the next piece are loaded one piece
+    // ahead, before the current piece issues its global atomics: a load
+    // issued after an atomic would wait for it (vector memory counts retire
     // in order).
+    constexpr int PR = kStripPieceRows / kWave;   // rows per lane in a piece
     const int vr0 = c_lo * kStripRows, vr1 = min(c_hi * kStripRows, rmax);
     const int w_lo = vr0 + (int)((long long)(vr1 - vr0) * wv / Cols::NWV);
     const int w_hi = vr0 + (int)((long long)(vr1 - vr0) * (wv + 1) / Cols::NWV);
-    auto load_row = [&](int p, double& xr, double& nr) {
-        const int j = p + lane;
-        xr = 0.0;
-        nr = 0.0;
-        if (j < min(p + kStripRows, w_hi)) {
-            xr = a.x[j];
-            nr = a.N[j];
+    // The loads are unconditional (the index clamped into the arrays; rows
+    // past the piece are masked where they are used): loads under a branch
+    // make the compiler wait for them at once, which undoes the prefetch.
+    auto load_rows = [&](int p, double (&xr)[PR], double (&nr)[PR]) {
+#pragma unroll
+        for (int t = 0; t < PR; ++t) {
+            const int j = min(p + lane + kWave * t, n - 1);
+            xr[t] = a.x[j];
+            nr[t] = a.N[j];
         }
     };
-    double xj_next, Nj_next;
-    load_row(w_lo, xj_next, Nj_next);
-    for (int rw0 = w_lo; rw0 < w_hi; rw0 += kStripRows) {
-        set_wave_priority(3 - 4 * (rw0 - w_lo) / (w_hi - w_lo));
-        const int rw1 = min(rw0 + kStripRows, w_hi);
-        // row factors of this piece: one row per lane
-        const int j = rw0 + lane;
-        const double xj = j < rw1 ? xj_next : 0.0;
-        const double Nj = Nj_next;
-        load_row(rw0 + kStripRows, xj_next, Nj_next);
-        double av[R];
-        if (j < rw1) {
+    auto pace = [&](int done) {
+        if (lane == 0) sm.prog[wv] = done;
+        wave_lds_sync();
+        int tot = 0;
+#pragma unroll
+        for (int w = 0; w < Cols::NWV; ++w) tot += sm.prog[w];
+        const int lead = done * Cols::NWV - tot;   // NWV x (own rows - mean rows)
+        constexpr int L = 16 * Cols::NWV;
+        set_wave_priority(lead > L ? 0 : lead > 0 ? 1 : lead > -L ? 2 : 3);
+    };
+    double xj_next[PR], Nj_next[PR];
+    load_rows(w_lo, xj_next, Nj_next);
+    for (int rw0 = w_lo; rw0 < w_hi; rw0 += kStripPieceRows) {
+        pace(rw0 - w_lo);
+        const int rw1 = min(rw0 + kStripPieceRows, w_hi);
+        double xj[PR], Nj[PR];
+#pragma unroll
+        for (int t = 0; t < PR; ++t) {
+            xj[t] = rw0 + lane + kWave * t < rw1 ? xj_next[t] : 0.0;
+            Nj[t] = Nj_next[t];
+        }
+        load_rows(rw0 + kStripPieceRows, xj_next, Nj_next);
+        // row factors of this piece (rows past its end: x = 1 for the roots,
+        // factors 0)
+        double av[PR][R];
+#pragma unroll
+        for (int t = 0; t < PR; ++t) {
+            const bool valid = rw0 + lane + kWave * t < rw1;
+            const double xr = valid ? xj[t] : 1.0;
             double cb, icb;
-            pivot_roots<KT>(xj, cb, icb);
-            K::row(av, xj, cb, icb, a.coef);
+            pivot_roots<KT>(xr, cb, icb);
+            K::row(av[t], xr, cb, icb, a.coef);
+            const double nr = valid ? Nj[t] : 0.0;
 #pragma unroll
-            for (int r = 0; r < R; ++r) av[r] *= Nj;
-            // death of row j against the block's columns (separable)
-            double dth = 0.0;
-#pragma unroll
-            for (int r = 0; r < R; ++r) dth = fma(av[r], TK[r], dth);
-            // issued after the next chunk's loads, so the pair loop below
-            // hides its latency
-            global_add(&a.rhs[j], -dth);
-        } else {
-#pragma unroll
-            for (int r = 0; r < R; ++r) av[r] = 0.0;
+            for (int r = 0; r < R; ++r) av[t][r] *= nr;
         }
 #pragma unroll
-        for (int r = 0; r < R; ++r) myrows[lane * RS + r] = av[r];
-        myrows[lane * RS + R] = xj;
+        for (int t = 0; t < PR; ++t) {
+            const int j = rw0 + lane + kWave * t;
+            if (j < rw1) {
+                // death of row j against the block's columns (separable),
+                // issued after the next piece's loads, so the pair loop
+                // below hides its latency
+                double dth = 0.0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) dth = fma(av[t][r], TK[r], dth);
+                global_add(&a.rhs[j], -dth);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) myrows[(lane + kWave * t) * RS + r] = av[t][r];
+            myrows[(lane + kWave * t) * RS + R] = xj[t];
+        }
         wave_lds_sync();
 
         const int nrows = rw1 - rw0;
-        const double xjmax = __shfl(xj, nrows - 1, kWave);
+        const int tl = (nrows - 1) / kWave;   // the piece's last row: lane (nrows-1) % 64, row tl
+        double xlast_row = xj[0];
+#pragma unroll
+        for (int t = 1; t < PR; ++t) xlast_row = tl == t ? xj[t] : xlast_row;
+        const double xjmax = __shfl(xlast_row, (nrows - 1) % kWave, kWave);
         bool ok = true;   // strip rows lie below the band: never on the diagonal
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -362,6 +413,7 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
             ld2(0, pa0, pa1);
             int jj = 0;
             for (; jj + 3 < nrows; jj += 4) {
+                if ((jj & 31) == 28) pace(rw0 - w_lo + jj);
                 ld2(jj + 2, pb0, pb1);
                 __builtin_amdgcn_sched_barrier(0);
                 fma2(pa0, pa1);
