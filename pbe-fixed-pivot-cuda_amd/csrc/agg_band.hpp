@@ -13,9 +13,8 @@
 //
 // A work item is one block and one segment of offsets; the segment is walked
 // as four interleaved streams (four independent FP64 chains per lane).  Row
-// factors A_r(j) and x_j of the band's rows sit in LDS as separate arrays, so
-// a lane's gather of row j = k - e reads consecutive addresses across the
-// wave.  Per-bracket sums are added to an LDS window of target bins, then
+// factors A_r(j) and x_j of the band's rows sit in LDS row by row, so a
+// lane's gather of row j = k - e is three 16-byte reads.  Per-bracket sums are added to an LDS window of target bins, then
 // flushed with global atomics.  The item of segment 0 also adds the band's
 // deaths (separable, as in the tile body).
 #pragma once
@@ -44,7 +43,11 @@ struct BandSegs {
 template <int KT>
 struct BandSmem {
     static constexpr int R = Kern<KT>::R;
-    double ra[R + 1][kBandRows];   // A_r(j) of the band rows, then x_j
+    // per band row: A_r(j) then x_j, an even number of doubles so a lane
+    // reads its row with 16-byte loads (a 48-byte stride at R = 5 puts the
+    // 16 lanes of an LDS cycle on distinct banks)
+    static constexpr int RSB = (R + 2) & ~1;
+    alignas(16) double ra[kBandRows][RSB];
     double colwin[kBandWin];
     double xw[kBandWin + 1];
     double idw[kBandWin];
@@ -114,10 +117,10 @@ __device__ __forceinline__ void agg_band_sweep(const AggArgs& a, const int q, co
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            sm.ra[r][i] = av[r];
+            sm.ra[i][r] = av[r];
             if (jlo + i < k0) tj[r] += av[r];
         }
-        sm.ra[R][i] = xr[t];
+        sm.ra[i][R] = xr[t];
     }
     // this thread's column factors B_r(k)
     double B[R];
@@ -178,7 +181,7 @@ __device__ __forceinline__ void agg_band_sweep(const AggArgs& a, const int q, co
             if (i < nrows) {
                 double dth = 0.0;
 #pragma unroll
-                for (int r = 0; r < R; ++r) dth = fma(sm.ra[r][i], TK[r], dth);
+                for (int r = 0; r < R; ++r) dth = fma(sm.ra[i][r], TK[r], dth);
                 put(jlo + i, -dth);
             }
         }
@@ -208,7 +211,7 @@ __device__ __forceinline__ void agg_band_sweep(const AggArgs& a, const int q, co
         for (int s = 0; s < NS; ++s) {
             const int es = min(e0 + (s + 1) * L - 1, min(e1 - 1, emax));
             const bool valid = col && es >= e0 + s * L;
-            vs[s] = valid ? sm.ra[R][k - es - jlo] + xk : 0.0;
+            vs[s] = valid ? sm.ra[k - es - jlo][R] + xk : 0.0;
             lo[s] = tid;
             // brackets beyond the window are left for the first step's search
             hi[s] = (valid && vs[s] >= xk1 && vs[s] < xlast && vs[s] < sm.xw[WIN]) ? WIN : tid + 1;
@@ -241,11 +244,19 @@ __device__ __forceinline__ void agg_band_sweep(const AggArgs& a, const int q, co
             const int e = e0 + (s + 1) * L - 1 - it;   // offsets run downward
             ok[s] = col && e < e1 && e <= emax;
             const int rr = ok[s] ? k - e - jlo : 0;
-            double p = sm.ra[0][rr] * B[0];
+            using D2 = double __attribute__((ext_vector_type(2)));
+            double rw[BandSmem<KT>::RSB];
 #pragma unroll
-            for (int r = 1; r < R; ++r) p = fma(sm.ra[r][rr], B[r], p);
+            for (int m = 0; m < BandSmem<KT>::RSB / 2; ++m) {
+                const D2 d2 = *reinterpret_cast<const D2*>(&sm.ra[rr][2 * m]);
+                rw[2 * m] = d2.x;
+                rw[2 * m + 1] = d2.y;
+            }
+            double p = rw[0] * B[0];
+#pragma unroll
+            for (int r = 1; r < R; ++r) p = fma(rw[r], B[r], p);
             pr[s] = e == 0 ? 0.5 * p : p;
-            vv[s] = sm.ra[R][rr] + xk;
+            vv[s] = rw[R] + xk;
         }
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
