@@ -197,7 +197,7 @@ static hipError_t launch_agg_t(AggArgs args, int split, hipStream_t stream)
     plan.nb = args.nb;
     plan.total = strip_total(plan);
     const long long G = resident_groups<KT, U>();
-    if (plan.total < G * StripWaves<U>::value / 4) {
+    if (plan.total < G) {
         // too little far work for the strips: one launch of all tiles
         const long long tiles = (long long)args.nb * (args.nb + 1) / 2;
         const long long items = band_items + tiles - band_tiles(args.nb);
