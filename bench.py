@@ -170,6 +170,40 @@ def rhs_variant(pbe_hip, hip, P, n, daughter, steps=20):
             "pair_updates_per_s": P.pair_units(n, nq) / (ms * 1e-3)}
 
 
+def deterministic_form(pbe_hip, hip, P, n=16384, steps=10):
+    """The atomic-free gather launches on the default workload: average step
+    time (zero rhs, aggregation, breakage) against the atomic launches."""
+    x, N, agg, brk, t_q, bw_q = P.config_flocculation(n)
+    nq = len(t_q)
+    dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
+    dt, db = hip.DeviceArray.from_host(t_q), hip.DeviceArray.from_host(bw_q)
+    drhs = hip.DeviceArray(n)
+    ap, bp = pbe_hip.agg_params(n, agg), pbe_hip.brk_params(n, nq, brk)
+    ws_bytes = max(pbe_hip.aggregation_deterministic_workspace_bytes(n),
+                   pbe_hip.breakage_deterministic_workspace_bytes(n))
+    ws = hip.DeviceArray(ws_bytes // 8 + 1)
+    s = hip.Stream()
+    rt = hip.runtime()
+
+    def step():
+        hip.check(rt.hipMemsetAsync(drhs.ptr, 0, drhs.nbytes, s.handle), "memset")
+        hip.check(pbe_hip.launch_aggregation_rhs_deterministic(dN, dx, drhs, ap, ws, s), "agg")
+        hip.check(pbe_hip.launch_breakage_rhs_deterministic(dN, dx, dt, db, drhs, bp, ws, s),
+                  "brk")
+
+    for _ in range(2):
+        step()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(s)
+    for _ in range(steps):
+        step()
+    e1.record(s)
+    s.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    return {"n": n, "ms_per_step": ms, "workspace_bytes": ws_bytes,
+            "pair_updates_per_s": P.pair_units(n, nq) / (ms * 1e-3)}
+
+
 def cpu_baseline(N, x, agg, brk, t_q, bw_q, budget_s):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
@@ -314,7 +348,10 @@ def main():
     out["variants"] = [rhs_variant(pbe_hip, hip, P, m, "symmetric") for m in (1024, 4096)]
     out["variants"].append(rhs_variant(pbe_hip, hip, P, n, "uniform"))
     if not args.no_forms:
-        out["forms"] = {"batched": batched_form(hip, P), "adjoint": adjoint_form(pbe_hip, hip, P)}
+        out["forms"] = {"batched": batched_form(hip, P), "adjoint": adjoint_form(pbe_hip, hip, P),
+                        "deterministic": deterministic_form(pbe_hip, hip, P, n)}
+        out["forms"]["deterministic"]["over_atomic"] = (
+            out["forms"]["deterministic"]["ms_per_step"] / ms_step)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, x, agg, brk, t_q, bw_q, args.cpu_seconds)
     print(json.dumps(out))
