@@ -332,7 +332,10 @@ def main():
             "step": "hipMemsetAsync(rhs) + aggregation launch + breakage launch",
         },
         "roofline": {
-            "bound": "fp64-valu",
+            # compute-bound; the pipe is FP64 VALU, whose MI355X peak equals
+            # the FP64 matrix peak (78.6 TFLOP/s)
+            "bound": "mfma",
+            "pipe": "fp64 VALU (no matrix instructions: not a contraction)",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
