@@ -124,19 +124,50 @@ __global__ __launch_bounds__(64) void det_agg_far_kernel(DetAggArgs a)
         idk = t[2 * R + 1];
         xk1 = k < n - 1 ? a.x[k + 1] : __builtin_inf();
     }
-    double accr = 0.0, accw = 0.0;
-    for (int j = r0; j < r1; ++j) {
-        const double* t = a.tab + (size_t)TS * j;   // the same row for every lane
+    // Rows are staged 64 at a time in LDS (one row's A_r and x per lane) and
+    // read back as wave-uniform broadcasts, four rows per step with their own
+    // accumulators (four independent FP64 chains per lane): row j goes to
+    // accumulator (j - r0) % 4, and the four are combined in a fixed order,
+    // so the sums depend only on n and the segment bounds.  A pair that is
+    // not far adds zero.
+    constexpr int FR = 4;
+    constexpr int RS = R + 1;
+    __shared__ double rows[kDetBlock][RS];
+    double ar[FR], aw[FR];
+#pragma unroll
+    for (int f = 0; f < FR; ++f) {
+        ar[f] = 0.0;
+        aw[f] = 0.0;
+    }
+    auto row = [&](int j, const double* t, double& acr, double& acw) {
         double pr = t[0] * B[0];
 #pragma unroll
         for (int r = 1; r < R; ++r) pr = fma(t[r], B[r], pr);
-        const double v = t[2 * R] + xk;
-        if (valid && j <= k && v < xk1) {
-            if (j == k) pr *= 0.5;
-            accr += pr;
-            accw = fma(v - xk, pr, accw);
+        const double v = t[R] + xk;
+        const bool far = valid && j <= k && v < xk1;
+        pr *= far ? (j == k ? 0.5 : 1.0) : 0.0;
+        acr += pr;
+        acw = fma(v - xk, pr, acw);
+    };
+    for (int jb = r0; jb < r1; jb += kDetBlock) {
+        const int nr = min(kDetBlock, r1 - jb);
+        if ((int)threadIdx.x < nr) {
+            const double* t = a.tab + (size_t)TS * (jb + threadIdx.x);
+#pragma unroll
+            for (int r = 0; r < R; ++r) rows[threadIdx.x][r] = t[r];
+            rows[threadIdx.x][R] = t[2 * R];
         }
+        __syncthreads();
+        int jj = 0;
+        for (; jj + FR <= nr; jj += FR) {
+#pragma unroll
+            for (int f = 0; f < FR; ++f) row(jb + jj + f, rows[jj + f], ar[f], aw[f]);
+        }
+        for (; jj < nr; ++jj) row(jb + jj, rows[jj], ar[jj % FR], aw[jj % FR]);
+        __syncthreads();
     }
+    const double accr = (ar[0] + ar[1]) + (ar[2] + ar[3]);
+    const double accw = (aw[0] + aw[1]) + (aw[2] + aw[3]);
     if (valid) {
         const double hi = accw * idk;   // 0 for k = n-1: every pair clips there
         a.part[(size_t)s * n + k] = accr - hi;
