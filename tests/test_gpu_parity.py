@@ -227,6 +227,24 @@ def test_strips_jittered_grid(pbe, oracle_mod):
     assert_m1(got, x, A, D, what="strips jittered n=12000")
 
 
+@pytest.mark.parametrize("r,kernel", [(1.001, 5), (1.001, 0), (1.005, 3)])
+def test_strips_fine_grid(pbe, oracle_mod, r, kernel):
+    """Geometric grids finer than the band (near reach 1000+ bins at
+    r = 1.001): every strip chunk takes the per-pair near path and most
+    brackets lie beyond the LDS windows (log-ratio guess plus fix-up)."""
+    from pbe_hip import problems as P
+    n = 5000
+    x = P.geometric_grid(n, 1e-18, r)
+    N = P.lognormal_ic(x, x[n // 2], 0.5, 1e12, noise=0.05, seed=int(r * 1e4) + kernel)
+    log_x0, inv_log_r = P.grid_lookup(x)
+    agg = dict(kernel=kernel, beta0=1e-17, beta_br=6.73e-18, beta_sh=1e-3, log_x0=log_x0,
+               inv_log_r=inv_log_r)
+    got = _gpu(pbe, N, x, agg=agg)
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg)
+    assert_parity(got, ref, A, what=f"fine grid r={r} kernel {kernel}")
+    assert_m1(got, x, A, D, what=f"fine grid r={r} kernel {kernel}")
+
+
 @pytest.mark.parametrize("kernel", range(6))
 def test_band_every_split(pbe, oracle_mod, kernel):
     """n = 9000: the column strips plus the diagonal band, for every band
