@@ -80,12 +80,19 @@ __device__ __forceinline__ void agg_band_sweep(const AggArgs& a, const int q, co
         xv[t] = (i <= WIN && b < n) ? a.x[b] : __builtin_inf();
         xn[t] = (i < WIN && b + 1 < n) ? a.x[b + 1] : __builtin_inf();
     }
+    // the segment's offsets, and the band rows its pairs use: rows
+    // k - e for k in the block and e in [e0, e1) (every row in segment 0,
+    // which also adds the band's deaths)
+    const int e0 = q == 0 ? segs.e0[seg] : segs.e[seg];
+    const int e1 = q == 0 ? segs.e0[seg + 1] : segs.e[seg + 1];
+    const int i_lo = seg == 0 ? 0 : max(0, k0 - e1 + 1 - jlo);
+    const int i_hi = seg == 0 ? nrows : min(nrows, min(k0 + T, n) - e0 - jlo);
     double xr[2], nr[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-        const int i = tid + 256 * t;
-        xr[t] = i < nrows ? a.x[jlo + i] : 0.0;
-        nr[t] = i < nrows ? a.N[jlo + i] : 0.0;
+        const int i = i_lo + tid + 256 * t;
+        xr[t] = i < i_hi ? a.x[jlo + i] : 0.0;
+        nr[t] = i < i_hi ? a.N[jlo + i] : 0.0;
     }
     const double nk = col ? a.N[k] : 0.0;
     const double xlast = a.x[n - 1];
@@ -103,24 +110,20 @@ __device__ __forceinline__ void agg_band_sweep(const AggArgs& a, const int q, co
     for (int r = 0; r < R; ++r) tj[r] = 0.0;
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-        const int i = tid + 256 * t;
-        double av[R];
-        if (i < nrows) {
+        const int i = i_lo + tid + 256 * t;
+        if (i < i_hi) {
+            double av[R];
             double cb, icb;
             pivot_roots<KT>(xr[t], cb, icb);
             K::row(av, xr[t], cb, icb, a.coef);
 #pragma unroll
-            for (int r = 0; r < R; ++r) av[r] *= nr[t];
-        } else {
-#pragma unroll
-            for (int r = 0; r < R; ++r) av[r] = 0.0;
+            for (int r = 0; r < R; ++r) {
+                av[r] *= nr[t];
+                sm.ra[i][r] = av[r];
+                if (jlo + i < k0) tj[r] += av[r];
+            }
+            sm.ra[i][R] = xr[t];
         }
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            sm.ra[i][r] = av[r];
-            if (jlo + i < k0) tj[r] += av[r];
-        }
-        sm.ra[i][R] = xr[t];
     }
     // this thread's column factors B_r(k)
     double B[R];
@@ -196,8 +199,6 @@ __device__ __forceinline__ void agg_band_sweep(const AggArgs& a, const int q, co
 
     // ---- births: the segment's offsets as four interleaved streams ----
     const int emax = k - jlo;                       // largest offset of this column
-    const int e0 = q == 0 ? segs.e0[seg] : segs.e[seg];
-    const int e1 = q == 0 ? segs.e0[seg + 1] : segs.e[seg + 1];
     const int L = (e1 - e0 + NS - 1) / NS;          // steps per stream
     // Each stream's bracket at its first pair, all four searches together:
     // a stream's first pair is where its bracket is furthest from (k, k+1),
