@@ -41,6 +41,8 @@ def parse():
                     help="wall-clock budget of the CPU baseline sample")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the batched and adjoint comparisons")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the n = 1024 / 4096 / GL16 variants (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run an elementwise device copy of BYTES (torch.neg, "
                          "reads and writes BYTES each), for scaling HBM byte counters")
@@ -348,8 +350,9 @@ def main():
         "ms_per_step_without_memset": ms_step_nomemset,
         "wall_s_timed_region": wall,
     }
-    out["variants"] = [rhs_variant(pbe_hip, hip, P, m, "symmetric") for m in (1024, 4096)]
-    out["variants"].append(rhs_variant(pbe_hip, hip, P, n, "uniform"))
+    if not args.no_variants:
+        out["variants"] = [rhs_variant(pbe_hip, hip, P, m, "symmetric") for m in (1024, 4096)]
+        out["variants"].append(rhs_variant(pbe_hip, hip, P, n, "uniform"))
     if not args.no_forms:
         out["forms"] = {"batched": batched_form(hip, P), "adjoint": adjoint_form(pbe_hip, hip, P),
                         "deterministic": deterministic_form(pbe_hip, hip, P, n)}
