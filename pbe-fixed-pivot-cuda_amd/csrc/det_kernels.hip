@@ -35,6 +35,7 @@ namespace pbe {
 namespace {
 
 constexpr int kDetBlock = 64;   // columns per far-partial item, bins per table block
+constexpr int kDetGather = 64;  // targets per gather workgroup (one wave: many workgroups)
 
 template <int KT>
 struct DetLayout {
@@ -230,33 +231,33 @@ __device__ __forceinline__ double det_near(const DetAggArgs& a, int Lb, bool upp
 }
 
 template <int KT>
-__global__ __launch_bounds__(256) void det_agg_gather_kernel(DetAggArgs a)
+__global__ __launch_bounds__(kDetGather) void det_agg_gather_kernel(DetAggArgs a)
 {
     using L = DetLayout<KT>;
     constexpr int R = L::R, TS = L::TS;
-    __shared__ double red[256];
+    __shared__ double red[kDetGather];
     __shared__ double TB[R];
     const int n = a.n, S = a.S;
     const int tid = threadIdx.x;
     // TB_r = sum of all column factors, fixed order (strided partials, then a tree)
     for (int r = 0; r < R; ++r) {
         double s = 0.0;
-        for (int b = tid; b < a.nbq; b += 256) s += a.tbq[(size_t)R * b + r];
+        for (int b = tid; b < a.nbq; b += kDetGather) s += a.tbq[(size_t)R * b + r];
         red[tid] = s;
         __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
+        for (int w = kDetGather / 2; w > 0; w >>= 1) {
             if (tid < w) red[tid] += red[tid + w];
             __syncthreads();
         }
         if (tid == 0) TB[r] = red[0];
         __syncthreads();
     }
-    const int i = blockIdx.x * 256 + tid;
+    const int i = blockIdx.x * kDetGather + tid;
     const double* x = a.x;
 
     // clipped pairs (v >= x_{n-1}, k <= n-2): the wave that owns bin n-1
     double clip = 0.0;
-    const int wave0 = blockIdx.x * 256 + (tid & ~63);
+    const int wave0 = blockIdx.x * kDetGather + (tid & ~63);
     if (wave0 <= n - 1 && n - 1 < wave0 + 64 && n >= 2) {
         const double xl = x[n - 1];
         const int lane = tid & 63;
@@ -289,7 +290,8 @@ hipError_t det_agg_launch(const DetAggArgs& a, hipStream_t s)
     const int nbq = a.nbq;
     hipLaunchKernelGGL(det_agg_table_kernel<KT>, dim3((unsigned)nbq), dim3(64), 0, s, a);
     hipLaunchKernelGGL(det_agg_far_kernel<KT>, dim3((unsigned)(nbq * a.S)), dim3(64), 0, s, a);
-    hipLaunchKernelGGL(det_agg_gather_kernel<KT>, dim3((unsigned)((a.n + 255) / 256)), dim3(256),
+    hipLaunchKernelGGL(det_agg_gather_kernel<KT>,
+                       dim3((unsigned)((a.n + kDetGather - 1) / kDetGather)), dim3(kDetGather),
                        0, s, a);
     return hipGetLastError();
 }
