@@ -83,6 +83,9 @@ struct AdjointStageFuse {
     double* av_next;
     const double* lam_base;
     double c1, c2;
+    double* lam_copy = nullptr;   // last stage: start of the next step
+    double* av_start = nullptr;
+    double c_start = 0.0;
 };
 hipError_t enqueue_vjp(const RhsProblem& p, const double* N, const double* lam,
                        const double* tab, double* gN, double* gth_rows, double* part,

@@ -43,10 +43,15 @@ struct VjpArgs {
     int log_scale;                    // 1: slots 0-3 hold d/d ln(coefficient)
     // Fused RK4 adjoint stage update (lam_acc null: plain VJP):
     //   lam_acc += gN;  av_next = c1 lam_base + c2 gN (when av_next non-null)
+    //   and, after the last stage of a step (lam_copy non-null), the start of
+    //   the next step: lam_copy = lam_acc, av_start = c_start lam_acc
     double* lam_acc;
     double* av_next;
     const double* lam_base;
     double c1, c2;
+    double* lam_copy;
+    double* av_start;
+    double c_start;
 };
 
 __global__ void vjp_table_kernel(const double* __restrict__ x, double* __restrict__ tab, int n)
@@ -224,6 +229,21 @@ __device__ __forceinline__ void vjp_part_body(const VjpArgs& a, const int m0, co
     }
 }
 
+// gN_m and the fused adjoint stage update of row m.
+__device__ __forceinline__ void vjp_store_row(const VjpArgs& a, const int m, const double gn)
+{
+    a.gN[m] = gn;
+    if (a.lam_acc) {
+        const double l = a.lam_acc[m] + gn;
+        a.lam_acc[m] = l;
+        if (a.av_next) a.av_next[m] = fma(a.c2, gn, a.c1 * a.lam_base[m]);
+        if (a.lam_copy) {
+            a.lam_copy[m] = l;
+            a.av_start[m] = a.c_start * l;
+        }
+    }
+}
+
 // Per row: sum the column-chunk partials in order, form lam . dR/dN_m
 // (aggregation + breakage) and the row's parameter partials.
 template <int KT>
@@ -276,33 +296,124 @@ __device__ __forceinline__ void vjp_finish_row(const VjpArgs& a, const int m,
             if (a.sel == 2) gr[4] += S * log(xm / a.v_ref) * Nm * h;
         }
     }
-    a.gN[m] = gn;
-    if (a.lam_acc) {
-        a.lam_acc[m] += gn;
-        if (a.av_next) a.av_next[m] = fma(a.c2, gn, a.c1 * a.lam_base[m]);
+    vjp_store_row(a, m, gn);
+}
+
+// Breakage gather of row m: sum_q bw_q eta(x_m t_q)(lam), one wave per row,
+// lane q taking the quadrature points q, q + 64, ... and the wave summing
+// them in the fixed butterfly order of wave_sum, so the bracket searches of a
+// row run side by side.  It depends on lam and the pivots only; the finish
+// pass applies S_m and the row guards.
+__device__ __forceinline__ double vjp_brk_gather_wave(const VjpArgs& a, const int m)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int n = a.n;
+    const double xm = a.x[m];
+    // fragments below x_{m+1} are bracketed within [0, m + 1]
+    const int mh = min(m + 1, n - 1);
+    const double xmh = a.x[mh];
+    double e = 0.0;
+    for (int q = lane; q < a.nq; q += kWave) {
+        const double v = xm * a.t_q[q];
+        double et;
+        if (v <= a.x[0]) et = a.lam[0];
+        else if (v >= a.x[n - 1]) et = a.lam[n - 1];
+        else {
+            const int hi = bracket_search(a.x, 0, v < xmh ? mh : n - 1, v), lo = hi - 1;
+            const double xl = a.x[lo], xh = a.x[hi];
+            const double w = (v - xl) / (xh - xl);
+            et = (1.0 - w) * a.lam[lo] + w * a.lam[hi];
+        }
+        e = fma(a.bw_q[q], et, e);
+    }
+    return wave_sum(e);
+}
+
+// Single instance: one wave per row.  Lane c adds the column-chunk partials
+// c, c + 64, ..., summed over the wave in wave_sum's fixed order; the
+// breakage gather of the row comes precomputed in brk_e.
+template <int KT>
+__device__ __forceinline__ void vjp_finish_row_wave(const VjpArgs& a, const int m,
+                                                    const double* __restrict__ part,
+                                                    int nchunk, size_t chunk_stride,
+                                                    const double* __restrict__ brk_e)
+{
+    constexpr int G = VjpSmem<KT>::G;
+    const int lane = threadIdx.x & (kWave - 1);
+    const double Nm = a.N[m], lm = a.lam[m], xm = a.x[m];
+    double gn = 0.0;
+    double gr[kNP] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if constexpr (KT >= 0) {
+        using VG = VjpGroups<KT>;
+        double acc[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) acc[g] = 0.0;
+        for (int c = lane; c < nchunk; c += kWave) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) acc[g] += part[c * chunk_stride + (size_t)G * m + g];
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const double t = wave_sum(acc[g]);
+            const double cf = VG::coef(g, a.coef);
+            gn = fma(cf, t, gn);
+            gr[VG::slot(g)] += (a.log_scale ? cf : 1.0) * (0.5 * Nm * t);
+        }
+    }
+    if (a.sel >= 0) {
+        const double S = selection_rt(a.sel, xm, a.S0, a.v_ref, a.alpha, a.v_min);
+        if (S != 0.0 && Nm > 2.2e-308) {
+            const double h = brk_e[m] - lm;
+            gn = fma(S, h, gn);
+            gr[3] += (a.log_scale ? a.S0 : 1.0) *
+                     (selection_shape(a.sel, xm, a.v_ref, a.alpha, a.v_min) * Nm * h);
+            if (a.sel == 2) gr[4] += S * log(xm / a.v_ref) * Nm * h;
+        }
+    }
+    if (lane == 0) {
+        double* grow = a.gth_rows + (size_t)kNP * m;
+#pragma unroll
+        for (int p = 0; p < kNP; ++p)
+            if (gr[p] != 0.0) grow[p] += gr[p];
+        vjp_store_row(a, m, gn);
     }
 }
 
-// Single instance.  Phase 1: grid (row blocks) x (column chunks).
+// Rows per breakage slice of the single-instance part launch: one per wave.
+constexpr int kVjpBrkRows = 256 / kWave;
+
+// Single instance.  Phase 1: grid (64-row blocks) x (nch column chunks, then
+// nbs breakage slices of kVjpBrkRows rows each), so the breakage gathers run
+// beside the aggregation chunks rather than after them.
 template <int KT>
 __global__ __launch_bounds__(256)
-void vjp_part_kernel(VjpArgs a, double* part, int kc)
+void vjp_part_kernel(VjpArgs a, double* part, int kc, int nch, double* brk_e)
 {
     __shared__ VjpSmem<KT> sm;
     constexpr int G = VjpSmem<KT>::G;
     const int c = blockIdx.y;
-    const int kb = c * kc, ke = min(a.n, kb + kc);
-    vjp_part_body<KT>(a, blockIdx.x * kWave, kb, ke, part + (size_t)c * G * a.n, sm);
+    if (c >= nch) {
+        const int m = blockIdx.x * kWave + (c - nch) * kVjpBrkRows + (int)(threadIdx.x / kWave);
+        if (m < a.n) {
+            const double e = vjp_brk_gather_wave(a, m);
+            if ((threadIdx.x & (kWave - 1)) == 0) brk_e[m] = e;
+        }
+        return;
+    }
+    if constexpr (KT >= 0) {
+        const int kb = c * kc, ke = min(a.n, kb + kc);
+        vjp_part_body<KT>(a, blockIdx.x * kWave, kb, ke, part + (size_t)c * G * a.n, sm);
+    }
 }
 
-// Phase 2: one thread per row.
+// Phase 2: one wave per row.
 template <int KT>
 __global__ __launch_bounds__(256)
-void vjp_finish_kernel(VjpArgs a, const double* part, int nchunk)
+void vjp_finish_kernel(VjpArgs a, const double* part, int nchunk, const double* brk_e)
 {
     constexpr int G = VjpSmem<KT>::G;
-    const int m = blockIdx.x * 256 + threadIdx.x;
-    if (m < a.n) vjp_finish_row<KT>(a, m, part, nchunk, (size_t)G * a.n);
+    const int m = blockIdx.x * (256 / kWave) + (int)(threadIdx.x / kWave);
+    if (m < a.n) vjp_finish_row_wave<KT>(a, m, part, nchunk, (size_t)G * a.n, brk_e);
 }
 
 // Batched.  Items are (instance, 64-row block); every instance is cut into
@@ -324,6 +435,10 @@ __device__ __forceinline__ VjpArgs instance_args(const VjpArgs& base, const Batc
     if (base.av_next) {
         a.av_next = base.av_next + d.off;
         a.lam_base = base.lam_base + d.off;
+    }
+    if (base.lam_copy) {
+        a.lam_copy = base.lam_copy + d.off;
+        a.av_start = base.av_start + d.off;
     }
     a.n = d.n;
     a.coef = KernelCoef{d.b0, d.b_br, d.b_sh};
@@ -391,21 +506,28 @@ static int vjp_chunks(int n)
     return std::max(c, 1);
 }
 
+// Single instance: nch chunks of up to 2 groups per row, then the breakage
+// gather of every row.
 size_t vjp_part_doubles(int n, bool batched)
 {
     return batched ? (size_t)kBatchVjpChunks * 2 * (size_t)n
-                   : (size_t)vjp_chunks(n) * 2 * (size_t)n;
+                   : ((size_t)vjp_chunks(n) * 2 + 1) * (size_t)n;
 }
 
 template <int KT>
 static void launch_vjp_single(const VjpArgs& a, double* part, hipStream_t s)
 {
-    const int nch = vjp_chunks(a.n);
-    const int kc = (a.n + nch - 1) / nch;
-    const dim3 g1((unsigned)((a.n + kWave - 1) / kWave), (unsigned)nch), b(256);
-    if (KT >= 0) hipLaunchKernelGGL(vjp_part_kernel<KT>, g1, b, 0, s, a, part, kc);
-    hipLaunchKernelGGL(vjp_finish_kernel<KT>, dim3((unsigned)((a.n + 255) / 256)), b, 0, s, a,
-                       (const double*)part, nch);
+    const int nch = KT >= 0 ? vjp_chunks(a.n) : 0;
+    const int kc = nch > 0 ? (a.n + nch - 1) / nch : 0;
+    const int nbs = a.sel >= 0 ? kWave / kVjpBrkRows : 0;
+    double* brk_e = part + (size_t)vjp_chunks(a.n) * 2 * (size_t)a.n;
+    const dim3 b(256);
+    if (nch + nbs > 0)
+        hipLaunchKernelGGL(vjp_part_kernel<KT>,
+                           dim3((unsigned)((a.n + kWave - 1) / kWave), (unsigned)(nch + nbs)),
+                           b, 0, s, a, part, kc, nch, brk_e);
+    hipLaunchKernelGGL(vjp_finish_kernel<KT>, dim3((unsigned)((a.n + 3) / 4)), b, 0, s, a,
+                       (const double*)part, nch, (const double*)brk_e);
 }
 
 template <int KT>
@@ -435,6 +557,9 @@ hipError_t enqueue_vjp(const RhsProblem& p, const double* N, const double* lam,
         a.lam_base = fuse->lam_base;
         a.c1 = fuse->c1;
         a.c2 = fuse->c2;
+        a.lam_copy = fuse->lam_copy;
+        a.av_start = fuse->av_start;
+        a.c_start = fuse->c_start;
     }
     a.N = N;
     a.x = p.x;
@@ -635,10 +760,13 @@ hipError_t enqueue_adjoint_backward(const RhsProblem& p, const double* gN_final,
             }
         }
         // reverse sweep over the segment; each VJP's finish pass also adds
-        // mu_s to the new lam and forms the next stage's cotangent
+        // mu_s to the new lam and forms the next stage's cotangent, and the
+        // last stage's finish starts the next step (lam2 <- lam, av0 <- dt/6 lam
+        // after the swap), so only the first step needs adj_start_kernel
         for (int i = s1 - 1; i >= s0; --i) {
             double* Yi = Y + nn * 4 * (size_t)(i - s0);
-            hipLaunchKernelGGL(adj_start_kernel, g, b, 0, s, lam, lam2, av0, n, dt / 6.0);
+            if (i == steps - 1)
+                hipLaunchKernelGGL(adj_start_kernel, g, b, 0, s, lam, lam2, av0, n, dt / 6.0);
             const AdjointStageFuse f4{lam2, av1, lam, dt / 3.0, dt};
             if ((e = enqueue_vjp(p, Yi + 3 * nn, av0, tab, mu, gth, part, s, &f4)) != hipSuccess)
                 return e;
@@ -648,7 +776,7 @@ hipError_t enqueue_adjoint_backward(const RhsProblem& p, const double* gN_final,
             const AdjointStageFuse f2{lam2, av1, lam, dt / 6.0, 0.5 * dt};
             if ((e = enqueue_vjp(p, Yi + nn, av0, tab, mu, gth, part, s, &f2)) != hipSuccess)
                 return e;
-            const AdjointStageFuse f1{lam2, nullptr, nullptr, 0.0, 0.0};
+            const AdjointStageFuse f1{lam2, nullptr, nullptr, 0.0, 0.0, lam, av0, dt / 6.0};
             if ((e = enqueue_vjp(p, Yi, av1, tab, mu, gth, part, s, &f1)) != hipSuccess) return e;
             double* t = lam;
             lam = lam2;
