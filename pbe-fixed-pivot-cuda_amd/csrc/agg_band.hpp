@@ -38,6 +38,13 @@ constexpr int kBandMaxSeg = 16;
 struct BandSegs {
     short e[kBandMaxSeg + 1];
     short e0[kBandMaxSeg + 1];
+    // On a geometric grid of ratio r, the bracket (c, c + 1) of x_{k-e} + x_k
+    // sits delta(e) = floor(log_r(1 + r^-e)) bins above k whatever k is.
+    // d[s][t] is delta at the first offset of stream t of segment s (d0: the
+    // same for block 0), -1 on a general grid; the kernel checks it against
+    // the pivots and falls back to a search where it does not hold.
+    short d[kBandMaxSeg][kBandStreams];
+    short d0[kBandMaxSeg][kBandStreams];
 };
 
 template <int KT>
@@ -208,16 +215,28 @@ __device__ __forceinline__ void agg_band_sweep(const AggArgs& a, const int q, co
     {
         int lo[NS], hi[NS];
         double vs[NS];
+        bool need = false;
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            const int es = min(e0 + (s + 1) * L - 1, min(e1 - 1, emax));
+            const int eu = min(e0 + (s + 1) * L - 1, e1 - 1);   // the stream's first offset
+            const int es = min(eu, emax);                       // (fewer rows in block 0)
             const bool valid = col && es >= e0 + s * L;
             vs[s] = valid ? sm.ra[k - es - jlo][R] + xk : 0.0;
             lo[s] = tid;
             // brackets beyond the window are left for the first step's search
             hi[s] = (valid && vs[s] >= xk1 && vs[s] < xlast && vs[s] < sm.xw[WIN]) ? WIN : tid + 1;
+            // geometric grid: try the bracket delta(e) bins above k first
+            const int dl = q == 0 ? segs.d0[seg][s] : segs.d[seg][s];
+            if (hi[s] - lo[s] > 1 && dl >= 0 && es == eu && tid + dl + 1 <= WIN) {
+                const int g = tid + dl;
+                if (sm.xw[g] <= vs[s] && vs[s] < sm.xw[g + 1]) {
+                    lo[s] = g;
+                    hi[s] = g + 1;
+                }
+            }
+            need = need || hi[s] - lo[s] > 1;
         }
-        for (int step = 0; step < 10; ++step) {
+        for (int step = 0; step < 10 && __any(need); ++step) {
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
                 if (hi[s] - lo[s] > 1) {
@@ -225,6 +244,9 @@ __device__ __forceinline__ void agg_band_sweep(const AggArgs& a, const int q, co
                     if (sm.xw[mid] <= vs[s]) lo[s] = mid; else hi[s] = mid;
                 }
             }
+            need = false;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) need = need || hi[s] - lo[s] > 1;
         }
 #pragma unroll
         for (int s = 0; s < NS; ++s) {

@@ -116,6 +116,22 @@ static BandSegs band_segments(int S, double inv_log_r)
             out[s] = (short)e;
         }
         for (int s = S + 1; s <= kBandMaxSeg; ++s) out[s] = (short)E;
+        // bracket offset at each stream's first pair (see BandSegs::d)
+        short(*dd)[kBandStreams] = corner ? sg.d0 : sg.d;
+        for (int s = 0; s < kBandMaxSeg; ++s) {
+            const int e0 = out[s], e1 = out[s + 1];
+            const int L = (e1 - e0 + kBandStreams - 1) / kBandStreams;
+            for (int t = 0; t < kBandStreams; ++t) {
+                const int eu = std::min(e0 + (t + 1) * L - 1, e1 - 1);
+                short dl = -1;
+                if (inv_log_r > 0.0 && s < S && eu >= e0 + t * L) {
+                    const double f =
+                        std::log1p(std::exp(-(double)eu / inv_log_r)) * inv_log_r;
+                    if (f >= 0.0 && f < 30000.0) dl = (short)std::floor(f);
+                }
+                dd[s][t] = dl;
+            }
+        }
     }
     return sg;
 }
