@@ -43,6 +43,13 @@ __device__ __forceinline__ void brk_block_body(const BrkArgs& a, const int p0, B
     const int wlo = wbase > 0 ? wbase : 0;  // first real bin of the window
     const int whi = min(wbase + W, n) - 1;  // last real bin of the window
 
+    // the parents' populations are loaded with the window, one round trip
+    double nj[(P + 255) / 256];
+#pragma unroll
+    for (int t = 0; t < (P + 255) / 256; ++t) {
+        const int j = p0 + tid + 256 * t;
+        nj[t] = j < n ? a.N[j] : 0.0;
+    }
     for (int i = tid; i < W; i += 256) {
         win[i] = 0.0;
         const int b = wbase + i;
@@ -57,12 +64,14 @@ __device__ __forceinline__ void brk_block_body(const BrkArgs& a, const int p0, B
     };
 
     // death and S_j N_j, once per parent
-    for (int i = tid; i < P; i += 256) {
+#pragma unroll
+    for (int t = 0; t < (P + 255) / 256; ++t) {
+        const int i = tid + 256 * t;
         const int j = p0 + i;
         double s = 0.0;
         if (j < n) {
             const double xj = xw[j - wbase];
-            const double Nj = a.N[j];
+            const double Nj = nj[t];
             const double Sj = selection<SEL>(xj, a.S0, a.v_ref, a.alpha, a.v_min);
             if (Sj != 0.0 && Nj > 2.2e-308) {
                 s = Sj * Nj;
@@ -93,6 +102,7 @@ __device__ __forceinline__ void brk_block_body(const BrkArgs& a, const int p0, B
             put(n - 1, c);
         } else {
             int hi;
+            double xl, xh;
             if (v >= xwlo && v < xwhi) {
                 // search the LDS copy: bracket inside [wlo, whi]
                 int l = wlo - wbase, h = whi - wbase;
@@ -101,11 +111,14 @@ __device__ __forceinline__ void brk_block_body(const BrkArgs& a, const int p0, B
                     if (xw[m] <= v) l = m; else h = m;
                 }
                 hi = h + wbase;
+                xl = xw[l];
+                xh = xw[h];
             } else {
                 hi = bracket_search(a.x, 0, n - 1, v);
+                xl = a.x[hi - 1];
+                xh = a.x[hi];
             }
             const int lo = hi - 1;
-            const double xl = a.x[lo], xh = a.x[hi];
             const double eta = (v - xl) / (xh - xl);
             put(lo, (1.0 - eta) * c);
             put(hi, eta * c);
