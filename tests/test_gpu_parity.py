@@ -227,6 +227,27 @@ def test_strips_jittered_grid(pbe, oracle_mod):
     assert_m1(got, x, A, D, what="strips jittered n=12000")
 
 
+@pytest.mark.parametrize("jitter,kernel", [(0.05, 5), (0.3, 5), (0.3, 0)])
+def test_geometric_lookup_on_jittered_grid(pbe, oracle_mod, jitter, kernel):
+    """A grid declared geometric (log_x0, inv_log_r of its nominal ratio 1.02)
+    whose section boundaries are jittered by up to `jitter` of a step: the
+    log-ratio guesses and the band's start-bracket offsets delta(e) then miss
+    in many lanes, and each miss must be caught against the pivots and fixed
+    up, so every pair still lands in its exact bracket (the oracle's exact
+    bracket mode)."""
+    from pbe_hip import problems as P
+    n = 9000
+    x = P.jittered_grid(n, 1e-18, 1.02, jitter, seed=int(jitter * 100) + kernel)
+    N = P.lognormal_ic(x, x[n // 3], 1.5, 1e12, noise=0.05, seed=77 + kernel)
+    log_x0, inv_log_r = np.log(1e-18 * np.sqrt(1.02)), 1.0 / np.log(1.02)
+    agg = dict(kernel=kernel, beta0=1e-17, beta_br=6.73e-18, beta_sh=1e-3, log_x0=log_x0,
+               inv_log_r=inv_log_r)
+    got = _gpu(pbe, N, x, agg=agg)
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg)
+    assert_parity(got, ref, A, what=f"declared geometric, jitter {jitter}, kernel {kernel}")
+    assert_m1(got, x, A, D, what=f"declared geometric, jitter {jitter}, kernel {kernel}")
+
+
 @pytest.mark.parametrize("r,kernel", [(1.001, 5), (1.001, 0), (1.005, 3)])
 def test_strips_fine_grid(pbe, oracle_mod, r, kernel):
     """Geometric grids finer than the band (near reach 1000+ bins at
