@@ -14,9 +14,11 @@
 //
 // The kernel is an atomic-free row gather: one lane owns row m, the four
 // waves of a workgroup take a quarter of the columns k each and are combined
-// in a fixed order, so results are bitwise repeatable.  Per-row parameter
-// partials are accumulated into a per-row array (each element owned by one
-// lane) and reduced in a fixed order at the end.
+// in a fixed order; a finish pass (one wave per row) sums the column chunks
+// and the breakage gather in a fixed order, so results are bitwise
+// repeatable.  Per-row parameter partials are accumulated into a per-row
+// array (each element written by one lane) and reduced in a fixed order at
+// the end.
 #include <algorithm>
 
 #include "batch_plan.hpp"
