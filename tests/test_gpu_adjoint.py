@@ -163,6 +163,31 @@ def test_batched_vjp_matches_single(pbe):
     assert np.allclose(gth, expect, rtol=1e-11, atol=0)
 
 
+@pytest.mark.parametrize("steps,every", [(1, 50), (5, 1), (3, 0)])
+def test_adjoint_short_runs(pbe, oracle_mod, steps, every):
+    """Edge cases of the reverse sweep: a single step (its last stage also
+    starts the step that never comes), a checkpoint interval beyond the step
+    count (one segment) and one step per segment."""
+    from pbe_hip import problems as P
+    x, N0, agg, brk, t_q, bw_q = _small_problem(96)
+    dt = 0.5
+    logT, mask, eps = _target(x, N0, agg, brk, t_q, bw_q, dt, steps, seed=2)
+    L, gN0, gth = _adjoint_grad(pbe, x, N0, agg, brk, t_q, bw_q, dt, steps, every, logT,
+                                mask, eps)
+    for p, slot in ((0, 1), (1, 2), (2, 3), (3, 4)):
+        e = np.zeros(5)
+        e[slot] = 1.0
+        Lc, dL = oracle_mod.cs_loss(N0, x, dt, steps, logT, mask, eps, agg, brk, t_q, bw_q,
+                                    dtheta=e, threads=4)
+        assert abs(Lc - L) <= 1e-10 * abs(L)
+        assert abs(gth[slot] - dL) <= 1e-8 * abs(dL), (P.FIT_PARAMS[p], steps, every)
+    rng = np.random.default_rng(5)
+    dN0 = rng.normal(size=len(N0)) * N0
+    _, dL = oracle_mod.cs_loss(N0, x, dt, steps, logT, mask, eps, agg, brk, t_q, bw_q,
+                               dN0=dN0, threads=4)
+    assert abs(gN0 @ dN0 - dL) <= 1e-8 * abs(dL), (steps, every)
+
+
 @pytest.mark.parametrize("every", [8, 0])
 def test_batched_adjoint_vs_complex_step(pbe, oracle_mod, every):
     """Joint fit of a small batch: d/d ln(scale) of the summed log misfit
