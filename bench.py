@@ -238,8 +238,8 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding is described in DESIGN.md)",
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 "
+                          "(the path is not sharded across GPUs here)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
