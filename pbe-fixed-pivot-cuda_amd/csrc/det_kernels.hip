@@ -130,7 +130,10 @@ __global__ __launch_bounds__(64) void det_agg_far_kernel(DetAggArgs a)
     // accumulators (four independent FP64 chains per lane): row j goes to
     // accumulator (j - r0) % 4, and the four are combined in a fixed order,
     // so the sums depend only on n and the segment bounds.  A pair that is
-    // not far adds zero.
+    // not far adds zero.  A staged block whose rows all lie below the
+    // column block and are far from every column (its last row is, since x
+    // increases) takes the same steps without the masks: the sums are
+    // bitwise those of the masked steps, which multiply by exactly 1.
     constexpr int FR = 4;
     constexpr int RS = R + 1;
     __shared__ double rows[kDetBlock][RS];
@@ -150,6 +153,14 @@ __global__ __launch_bounds__(64) void det_agg_far_kernel(DetAggArgs a)
         acr += pr;
         acw = fma(v - xk, pr, acw);
     };
+    auto row_far = [&](const double* t, double& acr, double& acw) {
+        double pr = t[0] * B[0];
+#pragma unroll
+        for (int r = 1; r < R; ++r) pr = fma(t[r], B[r], pr);
+        const double v = t[R] + xk;
+        acr += pr;
+        acw = fma(v - xk, pr, acw);
+    };
     for (int jb = r0; jb < r1; jb += kDetBlock) {
         const int nr = min(kDetBlock, r1 - jb);
         if ((int)threadIdx.x < nr) {
@@ -159,12 +170,22 @@ __global__ __launch_bounds__(64) void det_agg_far_kernel(DetAggArgs a)
             rows[threadIdx.x][R] = t[2 * R];
         }
         __syncthreads();
+        const bool allfar = jb + nr <= q * kDetBlock &&
+                            __all(!valid || rows[nr - 1][R] + xk < xk1);
         int jj = 0;
-        for (; jj + FR <= nr; jj += FR) {
+        if (allfar) {
+            for (; jj + FR <= nr; jj += FR) {
 #pragma unroll
-            for (int f = 0; f < FR; ++f) row(jb + jj + f, rows[jj + f], ar[f], aw[f]);
+                for (int f = 0; f < FR; ++f) row_far(rows[jj + f], ar[f], aw[f]);
+            }
+            for (; jj < nr; ++jj) row_far(rows[jj], ar[jj % FR], aw[jj % FR]);
+        } else {
+            for (; jj + FR <= nr; jj += FR) {
+#pragma unroll
+                for (int f = 0; f < FR; ++f) row(jb + jj + f, rows[jj + f], ar[f], aw[f]);
+            }
+            for (; jj < nr; ++jj) row(jb + jj, rows[jj], ar[jj % FR], aw[jj % FR]);
         }
-        for (; jj < nr; ++jj) row(jb + jj, rows[jj], ar[jj % FR], aw[jj % FR]);
         __syncthreads();
     }
     const double accr = (ar[0] + ar[1]) + (ar[2] + ar[3]);
