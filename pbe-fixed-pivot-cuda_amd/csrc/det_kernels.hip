@@ -36,6 +36,7 @@ namespace {
 
 constexpr int kDetBlock = 64;   // columns per far-partial item, bins per table block
 constexpr int kDetGather = 64;  // targets per gather workgroup (one wave: many workgroups)
+constexpr int kDetXBelow = 512; // pivots below a gather workgroup's targets held in LDS
 
 template <int KT>
 struct DetLayout {
@@ -221,30 +222,45 @@ __device__ __forceinline__ double det_rate(const double* tab, int j, int k)
     return j == k ? 0.5 * pr : pr;
 }
 
+// Pivot reads of the gather: from the workgroup's LDS copy of x[wlo, whi)
+// (LDS = true; every index its pairs touch lies there) or from global memory.
+template <bool LDS>
+struct DetXWin {
+    const double* xs;   // LDS copy of x[wlo, whi)
+    const double* x;
+    int wlo, whi;
+    __device__ __forceinline__ double operator()(int j) const
+    {
+        if constexpr (LDS) return lds_ld(&xs[j - wlo]);
+        else return glb_ld(&x[j]);
+    }
+};
+
 // Near births of target i from pairs whose bracket is (Lb, Lb+1); upper = true
 // takes the weight w of the upper pivot (Lb = i - 1), else 1 - w (Lb = i).
-template <int KT>
-__device__ __forceinline__ double det_near(const DetAggArgs& a, int Lb, bool upper)
+template <int KT, class XW>
+__device__ __forceinline__ double det_near(const DetAggArgs& a, const XW& X, int Lb,
+                                           bool upper)
 {
     using L = DetLayout<KT>;
     constexpr int R = L::R, TS = L::TS;
-    const double* x = a.x;
-    const double xL = x[Lb], xL1 = x[Lb + 1];
+    const double xL = X(Lb), xL1 = X(Lb + 1);
     const double idL = a.tab[(size_t)TS * Lb + 2 * R + 1];
     double acc = 0.0;
     int k = Lb - 1;
-    if (k < 0 || x[k] + x[k] < xL) return 0.0;
-    int ja = first_true(0, k + 1, [&](int j) { return x[j] + x[k] >= xL; });
-    int jb = first_true(ja, k + 1, [&](int j) { return x[j] + x[k] >= xL1; });
+    if (k < 0 || X(k) + X(k) < xL) return 0.0;
+    const double xk0 = X(k);
+    int ja = first_true(X.wlo, k + 1, [&](int j) { return X(j) + xk0 >= xL; });
+    int jb = first_true(ja, k + 1, [&](int j) { return X(j) + xk0 >= xL1; });
     for (; k >= 0 && ja <= k; --k) {
-        const double xk = x[k];
-        while (ja <= k && x[ja] + xk < xL) ++ja;
+        const double xk = X(k);
+        while (ja <= k && X(ja) + xk < xL) ++ja;
         if (jb < ja) jb = ja;
-        while (jb <= k && x[jb] + xk < xL1) ++jb;
+        while (jb <= k && X(jb) + xk < xL1) ++jb;
         const int je = min(jb, k + 1);   // rows j <= k
         for (int j = ja; j < je; ++j) {
             const double pr = det_rate<KT>(a.tab, j, k);
-            const double w = (x[j] + xk - xL) * idL;
+            const double w = (X(j) + xk - xL) * idL;
             acc += upper ? w * pr : pr - w * pr;
         }
     }
@@ -258,8 +274,15 @@ __global__ __launch_bounds__(kDetGather) void det_agg_gather_kernel(DetAggArgs a
     constexpr int R = L::R, TS = L::TS;
     __shared__ double red[kDetGather];
     __shared__ double TB[R];
+    __shared__ double xs[kDetXBelow + kDetGather + 1];
     const int n = a.n, S = a.S;
     const int tid = threadIdx.x;
+    // pivots of the neighbourhood: the targets' brackets and the rows and
+    // columns of their near pairs (on a grid of ratio 1.01 or more, all of
+    // them); the barriers below order these stores before the reads
+    const int i0 = blockIdx.x * kDetGather;
+    const int wlo = max(0, i0 - kDetXBelow), whi = min(n, i0 + kDetGather + 1);
+    for (int t = tid; t < whi - wlo; t += kDetGather) xs[t] = a.x[wlo + t];
     // TB_r = sum of all column factors, fixed order (strided partials, then a tree)
     for (int r = 0; r < R; ++r) {
         double s = 0.0;
@@ -274,20 +297,42 @@ __global__ __launch_bounds__(kDetGather) void det_agg_gather_kernel(DetAggArgs a
         __syncthreads();
     }
     const int i = blockIdx.x * kDetGather + tid;
-    const double* x = a.x;
+    // The window serves every pivot read when, for each bracket pivot L of
+    // the workgroup's targets (and n-1 for the clipped pairs), the smallest
+    // row of a pair reaching x_L, from column L-1 where the rows reach
+    // lowest, lies above wlo: x_wlo + x_{L-1} < x_L.  Columns and rows then
+    // stay in [wlo, L].
+    bool inwin = true;
+    if (wlo > 0) {
+        // lane t checks L = i0 - 1 + t and L + 1: the pivots [i0 - 1, i0 + 63]
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            const int L = min(i0 - 1 + tid + d, n - 1);
+            inwin = inwin && xs[0] + a.x[L - 1] < a.x[L];
+        }
+        if (i0 + kDetGather >= n - 1) inwin = inwin && xs[0] + a.x[n - 2] < a.x[n - 1];
+    }
+    inwin = __all(inwin);
+    const DetXWin<true> XL{xs, a.x, wlo, whi};
+    const DetXWin<false> XG{xs, a.x, 0, n};
 
     // clipped pairs (v >= x_{n-1}, k <= n-2): the wave that owns bin n-1
     double clip = 0.0;
     const int wave0 = blockIdx.x * kDetGather + (tid & ~63);
     if (wave0 <= n - 1 && n - 1 < wave0 + 64 && n >= 2) {
-        const double xl = x[n - 1];
-        const int lane = tid & 63;
-        for (int k = n - 2 - lane; k >= 0; k -= 64) {
-            if (x[k] + x[k] < xl) break;   // smaller k cannot reach x_{n-1}
-            const int jc = first_true(0, k + 1, [&](int j) { return x[j] + x[k] >= xl; });
-            for (int j = jc; j <= k; ++j) clip += det_rate<KT>(a.tab, j, k);
-        }
-        clip = wave_sum(clip);
+        auto clipped = [&](const auto& X) {
+            const double xl = X(n - 1);
+            const int lane = tid & 63;
+            double c = 0.0;
+            for (int k = n - 2 - lane; k >= X.wlo; k -= 64) {
+                const double xk = X(k);
+                if (xk + xk < xl) break;   // smaller k cannot reach x_{n-1}
+                const int jc = first_true(X.wlo, k + 1, [&](int j) { return X(j) + xk >= xl; });
+                for (int j = jc; j <= k; ++j) c += det_rate<KT>(a.tab, j, k);
+            }
+            return c;
+        };
+        clip = wave_sum(inwin ? clipped(XL) : clipped(XG));
     }
     if (i >= n) return;
 
@@ -295,8 +340,13 @@ __global__ __launch_bounds__(kDetGather) void det_agg_gather_kernel(DetAggArgs a
     for (int s = 0; s < S; ++s) acc += a.part[(size_t)s * n + i];
     if (i >= 1)
         for (int s = 0; s < S; ++s) acc += a.part[(size_t)(S + s) * n + i - 1];
-    if (i <= n - 2) acc += det_near<KT>(a, i, false);
-    if (i >= 1 && i - 1 <= n - 2) acc += det_near<KT>(a, i - 1, true);
+    if (inwin) {
+        if (i <= n - 2) acc += det_near<KT>(a, XL, i, false);
+        if (i >= 1 && i - 1 <= n - 2) acc += det_near<KT>(a, XL, i - 1, true);
+    } else {
+        if (i <= n - 2) acc += det_near<KT>(a, XG, i, false);
+        if (i >= 1 && i - 1 <= n - 2) acc += det_near<KT>(a, XG, i - 1, true);
+    }
     if (i == n - 1) acc += clip;
     const double* t = a.tab + (size_t)TS * i;
     double dth = 0.0;
