@@ -35,7 +35,9 @@ namespace pbe {
 namespace {
 
 constexpr int kDetBlock = 64;   // columns per far-partial item, bins per table block
-constexpr int kDetGather = 64;  // targets per gather workgroup (one wave: many workgroups)
+constexpr int kDetGather = 64;  // threads per gather workgroup (one wave: many workgroups)
+constexpr int kDetSplit = 4;    // lanes per gather target (its near columns interleaved)
+constexpr int kDetTargets = kDetGather / kDetSplit;   // targets per gather workgroup
 constexpr int kDetXBelow = 512; // pivots below a gather workgroup's targets held in LDS
 
 template <int KT>
@@ -238,21 +240,24 @@ struct DetXWin {
 
 // Near births of target i from pairs whose bracket is (Lb, Lb+1); upper = true
 // takes the weight w of the upper pivot (Lb = i - 1), else 1 - w (Lb = i).
+// Columns k = Lb - 1 - sub, Lb - 1 - sub - kDetSplit, ... (this lane's share
+// of the target's columns, walked downward): the rows of a column whose sum
+// lands in the bracket form a range [ja, jb) that only moves up as k falls.
 template <int KT, class XW>
 __device__ __forceinline__ double det_near(const DetAggArgs& a, const XW& X, int Lb,
-                                           bool upper)
+                                           bool upper, int sub)
 {
     using L = DetLayout<KT>;
     constexpr int R = L::R, TS = L::TS;
     const double xL = X(Lb), xL1 = X(Lb + 1);
     const double idL = a.tab[(size_t)TS * Lb + 2 * R + 1];
     double acc = 0.0;
-    int k = Lb - 1;
-    if (k < 0 || X(k) + X(k) < xL) return 0.0;
+    int k = Lb - 1 - sub;
+    if (k < 0 || X(k) + X(k) < xL) return 0.0;   // nor can any smaller column
     const double xk0 = X(k);
     int ja = first_true(X.wlo, k + 1, [&](int j) { return X(j) + xk0 >= xL; });
     int jb = first_true(ja, k + 1, [&](int j) { return X(j) + xk0 >= xL1; });
-    for (; k >= 0 && ja <= k; --k) {
+    for (; k >= 0 && ja <= k; k -= kDetSplit) {
         const double xk = X(k);
         while (ja <= k && X(ja) + xk < xL) ++ja;
         if (jb < ja) jb = ja;
@@ -274,14 +279,14 @@ __global__ __launch_bounds__(kDetGather) void det_agg_gather_kernel(DetAggArgs a
     constexpr int R = L::R, TS = L::TS;
     __shared__ double red[kDetGather];
     __shared__ double TB[R];
-    __shared__ double xs[kDetXBelow + kDetGather + 1];
+    __shared__ double xs[kDetXBelow + kDetTargets + 1];
     const int n = a.n, S = a.S;
     const int tid = threadIdx.x;
     // pivots of the neighbourhood: the targets' brackets and the rows and
     // columns of their near pairs (on a grid of ratio 1.01 or more, all of
     // them); the barriers below order these stores before the reads
-    const int i0 = blockIdx.x * kDetGather;
-    const int wlo = max(0, i0 - kDetXBelow), whi = min(n, i0 + kDetGather + 1);
+    const int i0 = blockIdx.x * kDetTargets;
+    const int wlo = max(0, i0 - kDetXBelow), whi = min(n, i0 + kDetTargets + 1);
     for (int t = tid; t < whi - wlo; t += kDetGather) xs[t] = a.x[wlo + t];
     // TB_r = sum of all column factors, fixed order (strided partials, then a tree)
     for (int r = 0; r < R; ++r) {
@@ -296,7 +301,8 @@ __global__ __launch_bounds__(kDetGather) void det_agg_gather_kernel(DetAggArgs a
         if (tid == 0) TB[r] = red[0];
         __syncthreads();
     }
-    const int i = blockIdx.x * kDetGather + tid;
+    // lanes 4m .. 4m + 3 share target i0 + m
+    const int i = i0 + tid / kDetSplit, sub = tid % kDetSplit;
     // The window serves every pivot read when, for each bracket pivot L of
     // the workgroup's targets (and n-1 for the clipped pairs), the smallest
     // row of a pair reaching x_L, from column L-1 where the rows reach
@@ -304,27 +310,22 @@ __global__ __launch_bounds__(kDetGather) void det_agg_gather_kernel(DetAggArgs a
     // stay in [wlo, L].
     bool inwin = true;
     if (wlo > 0) {
-        // lane t checks L = i0 - 1 + t and L + 1: the pivots [i0 - 1, i0 + 63]
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-            const int L = min(i0 - 1 + tid + d, n - 1);
-            inwin = inwin && xs[0] + a.x[L - 1] < a.x[L];
-        }
-        if (i0 + kDetGather >= n - 1) inwin = inwin && xs[0] + a.x[n - 2] < a.x[n - 1];
+        // lane t <= kDetTargets checks L = i0 - 1 + t: the pivots [i0 - 1, i0 + 15]
+        const int L = min(i0 - 1 + min(tid, kDetTargets), n - 1);
+        inwin = xs[0] + a.x[L - 1] < a.x[L];
+        if (i0 + kDetTargets >= n - 1) inwin = inwin && xs[0] + a.x[n - 2] < a.x[n - 1];
     }
     inwin = __all(inwin);
     const DetXWin<true> XL{xs, a.x, wlo, whi};
     const DetXWin<false> XG{xs, a.x, 0, n};
 
-    // clipped pairs (v >= x_{n-1}, k <= n-2): the wave that owns bin n-1
+    // clipped pairs (v >= x_{n-1}, k <= n-2): the workgroup that owns bin n-1
     double clip = 0.0;
-    const int wave0 = blockIdx.x * kDetGather + (tid & ~63);
-    if (wave0 <= n - 1 && n - 1 < wave0 + 64 && n >= 2) {
+    if (i0 <= n - 1 && n - 1 < i0 + kDetTargets && n >= 2) {
         auto clipped = [&](const auto& X) {
             const double xl = X(n - 1);
-            const int lane = tid & 63;
             double c = 0.0;
-            for (int k = n - 2 - lane; k >= X.wlo; k -= 64) {
+            for (int k = n - 2 - tid; k >= X.wlo; k -= kDetGather) {
                 const double xk = X(k);
                 if (xk + xk < xl) break;   // smaller k cannot reach x_{n-1}
                 const int jc = first_true(X.wlo, k + 1, [&](int j) { return X(j) + xk >= xl; });
@@ -334,19 +335,32 @@ __global__ __launch_bounds__(kDetGather) void det_agg_gather_kernel(DetAggArgs a
         };
         clip = wave_sum(inwin ? clipped(XL) : clipped(XG));
     }
-    if (i >= n) return;
+
+    // near births: each lane its share of the target's columns, the four
+    // shares summed in a fixed order ((s0 + s1) + (s2 + s3), the same value
+    // in all four lanes)
+    double nl = 0.0, nu = 0.0;
+    if (i < n) {
+        if (inwin) {
+            if (i <= n - 2) nl = det_near<KT>(a, XL, i, false, sub);
+            if (i >= 1 && i - 1 <= n - 2) nu = det_near<KT>(a, XL, i - 1, true, sub);
+        } else {
+            if (i <= n - 2) nl = det_near<KT>(a, XG, i, false, sub);
+            if (i >= 1 && i - 1 <= n - 2) nu = det_near<KT>(a, XG, i - 1, true, sub);
+        }
+    }
+    nl += dpp_move<0xB1, 0xF>(nl);   // quad_perm 1,0,3,2
+    nl += dpp_move<0x4E, 0xF>(nl);   // quad_perm 2,3,0,1
+    nu += dpp_move<0xB1, 0xF>(nu);
+    nu += dpp_move<0x4E, 0xF>(nu);
+    if (i >= n || sub != 0) return;
 
     double acc = 0.0;
     for (int s = 0; s < S; ++s) acc += a.part[(size_t)s * n + i];
     if (i >= 1)
         for (int s = 0; s < S; ++s) acc += a.part[(size_t)(S + s) * n + i - 1];
-    if (inwin) {
-        if (i <= n - 2) acc += det_near<KT>(a, XL, i, false);
-        if (i >= 1 && i - 1 <= n - 2) acc += det_near<KT>(a, XL, i - 1, true);
-    } else {
-        if (i <= n - 2) acc += det_near<KT>(a, XG, i, false);
-        if (i >= 1 && i - 1 <= n - 2) acc += det_near<KT>(a, XG, i - 1, true);
-    }
+    acc += nl;
+    acc += nu;
     if (i == n - 1) acc += clip;
     const double* t = a.tab + (size_t)TS * i;
     double dth = 0.0;
@@ -362,8 +376,8 @@ hipError_t det_agg_launch(const DetAggArgs& a, hipStream_t s)
     hipLaunchKernelGGL(det_agg_table_kernel<KT>, dim3((unsigned)nbq), dim3(64), 0, s, a);
     hipLaunchKernelGGL(det_agg_far_kernel<KT>, dim3((unsigned)(nbq * a.S)), dim3(64), 0, s, a);
     hipLaunchKernelGGL(det_agg_gather_kernel<KT>,
-                       dim3((unsigned)((a.n + kDetGather - 1) / kDetGather)), dim3(kDetGather),
-                       0, s, a);
+                       dim3((unsigned)((a.n + kDetTargets - 1) / kDetTargets)),
+                       dim3(kDetGather), 0, s, a);
     return hipGetLastError();
 }
 
