@@ -38,6 +38,7 @@ constexpr int kDetBlock = 64;   // columns per far-partial item, bins per table 
 constexpr int kDetGather = 64;  // threads per gather workgroup (one wave: many workgroups)
 constexpr int kDetSplit = 4;    // lanes per gather target (its near columns interleaved)
 constexpr int kDetTargets = kDetGather / kDetSplit;   // targets per gather workgroup
+static_assert(kDetGather == kWave, "the gather's sums are wave butterflies");
 constexpr int kDetXBelow = 512; // pivots below a gather workgroup's targets held in LDS
 
 template <int KT>
@@ -277,8 +278,6 @@ __global__ __launch_bounds__(kDetGather) void det_agg_gather_kernel(DetAggArgs a
 {
     using L = DetLayout<KT>;
     constexpr int R = L::R, TS = L::TS;
-    __shared__ double red[kDetGather];
-    __shared__ double TB[R];
     __shared__ double xs[kDetXBelow + kDetTargets + 1];
     const int n = a.n, S = a.S;
     const int tid = threadIdx.x;
@@ -288,19 +287,21 @@ __global__ __launch_bounds__(kDetGather) void det_agg_gather_kernel(DetAggArgs a
     const int i0 = blockIdx.x * kDetTargets;
     const int wlo = max(0, i0 - kDetXBelow), whi = min(n, i0 + kDetTargets + 1);
     for (int t = tid; t < whi - wlo; t += kDetGather) xs[t] = a.x[wlo + t];
-    // TB_r = sum of all column factors, fixed order (strided partials, then a tree)
-    for (int r = 0; r < R; ++r) {
-        double s = 0.0;
-        for (int b = tid; b < a.nbq; b += kDetGather) s += a.tbq[(size_t)R * b + r];
-        red[tid] = s;
-        __syncthreads();
-        for (int w = kDetGather / 2; w > 0; w >>= 1) {
-            if (tid < w) red[tid] += red[tid + w];
-            __syncthreads();
-        }
-        if (tid == 0) TB[r] = red[0];
-        __syncthreads();
+    // TB_r = sum of all column factors in a fixed order (each lane its
+    // strided block totals in turn, then the wave's butterfly), the same in
+    // every lane and every workgroup
+    double TB[R];
+    {
+        double sb[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) sb[r] = 0.0;
+        for (int b = tid; b < a.nbq; b += kDetGather)
+#pragma unroll
+            for (int r = 0; r < R; ++r) sb[r] += a.tbq[(size_t)R * b + r];
+#pragma unroll
+        for (int r = 0; r < R; ++r) TB[r] = wave_sum(sb[r]);
     }
+    __syncthreads();   // the window's stores before its reads
     // lanes 4m .. 4m + 3 share target i0 + m
     const int i = i0 + tid / kDetSplit, sub = tid % kDetSplit;
     // The window serves every pivot read when, for each bracket pivot L of
