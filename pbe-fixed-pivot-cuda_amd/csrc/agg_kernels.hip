@@ -40,6 +40,7 @@
 //    with FP64 global atomics (zero entries skipped).
 #include <atomic>
 #include <cmath>
+#include <cstring>
 
 #include "agg_band.hpp"
 #include "agg_strip.hpp"
@@ -136,6 +137,22 @@ static BandSegs band_segments(int S, double inv_log_r)
     return sg;
 }
 
+// band_segments costs a few hundred log/exp evaluations (about 9 us of host
+// time); a launch reuses the calling thread's last plan when S and the grid
+// ratio are the same (compared bit for bit).
+static const BandSegs& band_segments_cached(int S, double inv_log_r)
+{
+    thread_local int c_S = -1;
+    thread_local double c_ilr = 0.0;
+    thread_local BandSegs c_segs;
+    if (S != c_S || std::memcmp(&inv_log_r, &c_ilr, sizeof(double)) != 0) {
+        c_segs = band_segments(S, inv_log_r);
+        c_S = S;
+        c_ilr = inv_log_r;
+    }
+    return c_segs;
+}
+
 // Small problems (too few far chunks to fill the device once): one launch of
 // every tile, the band slices first, then the far tiles whole in
 // diagonal-major order, all scheduled by the hardware.
@@ -226,7 +243,7 @@ static hipError_t launch_agg_t(AggArgs args, int split, hipStream_t stream)
     if (T == kBandT) {
         // band as diagonal sweeps: split = offset segments per block
         hipLaunchKernelGGL((agg_sweep_kernel<KT>), dim3((unsigned)(args.nb * split)), dim3(256),
-                           0, stream, args, split, band_segments(split, args.inv_log_r));
+                           0, stream, args, split, band_segments_cached(split, args.inv_log_r));
     } else {
         hipLaunchKernelGGL((agg_band_kernel<KT, U>), dim3((unsigned)band_items), dim3(256), 0,
                            stream, args, split);
