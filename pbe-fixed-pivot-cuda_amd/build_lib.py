@@ -85,7 +85,8 @@ def build_host_checks():
 
 def build_bench_helpers():
     """bench/batched_loop.cpp: batched launch vs a loop of plain calls;
-    bench/fp64_issue.hip: FP64 FMA issue rates (waves per SIMD x chains)."""
+    bench/fp64_issue.hip: FP64 FMA issue rates (waves per SIMD x chains);
+    bench/atomic_rates.hip: LDS and global FP64 atomic-add rates."""
     src = os.path.join(HERE, "bench", "batched_loop.cpp")
     out = os.path.join(HERE, "bench", "batched_loop")
     _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", src, "-o", out + ".tmp",
@@ -95,6 +96,10 @@ def build_bench_helpers():
     fp = os.path.join(HERE, "bench", "fp64_issue")
     _run([HIPCC, *COMMON, src, "-o", fp + ".tmp"])
     os.replace(fp + ".tmp", fp)
+    src = os.path.join(HERE, "bench", "atomic_rates.hip")
+    ar = os.path.join(HERE, "bench", "atomic_rates")
+    _run([HIPCC, *COMMON, src, "-o", ar + ".tmp"])
+    os.replace(ar + ".tmp", ar)
     return out
 
 
