@@ -221,6 +221,18 @@ def cpu_baseline(N, x, agg, brk, t_q, bw_q, budget_s):
             break
     n, nq = len(N), len(t_q)
     units = n * (n + 1) // 2 + n * nq
+    # the same on one core: one evaluation
+    t1 = time.perf_counter()
+    r = oracle.port_aggregation_rhs(N, x, threads=1, **agg)
+    r += oracle.port_breakage_rhs(N, x, t_q, bw_q, threads=1, **brk)
+    el1 = time.perf_counter() - t1
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f
+                          if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": units * evals / el,
         "unit": "pair-updates/s",
@@ -229,6 +241,10 @@ def cpu_baseline(N, x, agg, brk, t_q, bw_q, budget_s):
         "sample": (f"{evals} full agg+breakage RHS evaluation(s) at n={n} "
                    f"({el:.2f} s wall, {threads} OpenMP threads, "
                    "C restatement of the reference pair loop)"),
+        "one_core_value": units / el1,
+        "one_core_sample": f"1 evaluation on 1 thread ({el1:.2f} s wall)",
+        "cpu_model": model,
+        "host_cpus_visible": os.cpu_count(),
     }
 
 
@@ -288,7 +304,9 @@ def main():
     hip.synchronize()
     wall = time.perf_counter() - t0
     ms_step = e0.elapsed_ms(e1) / K
-    agg_ms = sum(a.elapsed_ms(b) for a, b in ev) / K
+    agg_each = sorted(a.elapsed_ms(b) for a, b in ev)
+    agg_ms = sum(agg_each) / K
+    agg_median = agg_each[K // 2] if K % 2 else 0.5 * (agg_each[K // 2 - 1] + agg_each[K // 2])
 
     # same K steps without the per-step memset (rhs accumulates; timing only)
     f0, f1 = hip.Event(), hip.Event()
@@ -346,6 +364,7 @@ def main():
             "kernel": "aggregation call: agg_sweep_kernel + agg_strip_kernel <BrownianShear>",
             "flop_per_launch": flops,
             "avg_launch_ms": agg_ms,
+            "median_launch_ms": agg_median,
         },
         "ms_per_step_without_memset": ms_step_nomemset,
         "wall_s_timed_region": wall,
