@@ -28,6 +28,22 @@ def test_smoluchowski_rk4_closed_form(pbe, oracle_mod):
     assert err <= 1e-11
 
 
+@pytest.mark.parametrize("case", ["sum", "product", "breakage"])
+def test_rk4_moment_laws(pbe, case):
+    """Device RK4 against the discrete moment laws: the sum kernel's M0
+    decays by RK4's own polynomial, the product kernel's and linear-selection
+    breakage's M0 is linear in t, M1 is conserved (Golovin, product-kernel
+    and Ziff-McGrady moment behaviour of the fixed-pivot system)."""
+    from pbe_hip import hip
+    import moment_cases as MC
+    x, N0, kw, M0_expected, M1 = MC.CASES[case]()
+    it = pbe.Integrator(x, **kw)
+    dN = hip.DeviceArray.from_host(N0)
+    hip.check(it.step(dN, MC.DT, MC.STEPS))
+    hip.synchronize()
+    MC.check(x, dN.download(), M0_expected, M1, tol_m0=1e-11)
+
+
 def test_flocculation_rk4_vs_oracle(pbe, oracle_mod):
     from pbe_hip import hip, problems as P
     x, N0, agg, brk, _, _ = P.config_adjoint(512)

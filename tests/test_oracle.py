@@ -7,7 +7,8 @@ sign, wrong bracket index, swapped split weights) fails at least one:
   * exact discrete identities of the semi-discrete fixed-pivot system;
   * the reference's own known answers and invariants
     (tests/unit/test_aggregation.cpp, tests/unit/test_breakage.cpp);
-  * the Smoluchowski closed form for the constant kernel under RK4;
+  * the Smoluchowski closed form for the constant kernel under RK4, and the
+    sum-kernel, product-kernel and linear-selection moment laws under RK4;
   * the reference bracket rule (log-ratio formula) against the exact bracket.
 """
 import json
@@ -233,3 +234,14 @@ def test_port_matches_oracle(oracle_mod):
     pa = oracle_mod.port_aggregation_rhs(N, x, threads=4, **agg)
     pb = oracle_mod.port_breakage_rhs(N, x, t_q, bw_q, threads=4, **brk)
     assert_parity(pa + pb, ra + rb, Aa + Ab, tol=1e-13, what="port")
+
+
+@pytest.mark.parametrize("case", ["sum", "product", "breakage"])
+def test_oracle_rk4_moment_laws(oracle_mod, case):
+    """The oracle's RK4 against the discrete moment laws (sum kernel: M0
+    decays by RK4's own polynomial; product kernel and linear-selection
+    breakage: M0 linear in t; M1 conserved)."""
+    import moment_cases as MC
+    x, N0, kw, M0_expected, M1 = MC.CASES[case]()
+    N = oracle_mod.rk4(N0, x, MC.DT, MC.STEPS, **kw)
+    MC.check(x, N, M0_expected, M1, tol_m0=1e-12)
