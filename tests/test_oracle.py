@@ -6,7 +6,8 @@ sign, wrong bracket index, swapped split weights) fails at least one:
     with hat functions and a full-row death sum (tests/golden);
   * exact discrete identities of the semi-discrete fixed-pivot system;
   * the reference's own known answers and invariants
-    (tests/unit/test_aggregation.cpp, tests/unit/test_breakage.cpp);
+    (tests/unit/test_aggregation.cpp, tests/unit/test_breakage.cpp) and the
+    errors its five example programs document (docs/examples.md);
   * the Smoluchowski closed form for the constant kernel under RK4, and the
     sum-kernel, product-kernel and linear-selection moment laws under RK4;
   * the reference bracket rule (log-ratio formula) against the exact bracket.
@@ -245,3 +246,31 @@ def test_oracle_rk4_moment_laws(oracle_mod, case):
     x, N0, kw, M0_expected, M1 = MC.CASES[case]()
     N = oracle_mod.rk4(N0, x, MC.DT, MC.STEPS, **kw)
     MC.check(x, N, M0_expected, M1, tol_m0=1e-12)
+
+
+def _oracle_euler(orc, case):
+    """Explicit Euler as examples/common/pbe_examples_utils.hpp:391-407:
+    rhs from zero, then N_i += dt * rhs_i."""
+    N = np.array(case["N0"], dtype=float)
+    kw = case["kw"]
+    for _ in range(case["steps"]):
+        r = np.zeros_like(N)
+        if "agg" in kw:
+            r += orc.aggregation_rhs(N, case["x"], **kw["agg"])[0]
+        if "brk" in kw:
+            r += orc.breakage_rhs(N, case["x"], kw["t_q"], kw["bw_q"], **kw["brk"])[0]
+        N = N + case["dt"] * r
+    return N
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+def test_oracle_reference_examples(oracle_mod, k):
+    """The reference's five example programs (docs/examples.md): the oracle
+    reproduces their documented errors against the analytic moments."""
+    import reference_examples as RE
+    c = RE.example(k)
+    if c["method"] == "rk4":
+        N = oracle_mod.rk4(c["N0"], c["x"], c["dt"], c["steps"], **c["kw"])
+    else:
+        N = _oracle_euler(oracle_mod, c)
+    print(k, RE.assert_matches(c, N))
