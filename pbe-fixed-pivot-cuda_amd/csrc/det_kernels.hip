@@ -40,6 +40,7 @@ constexpr int kDetSplit = 4;    // lanes per gather target (its near columns int
 constexpr int kDetTargets = kDetGather / kDetSplit;   // targets per gather workgroup
 static_assert(kDetGather == kWave, "the gather's sums are wave butterflies");
 constexpr int kDetXBelow = 512; // pivots below a gather workgroup's targets held in LDS
+constexpr int kDetTabBelow = 256; // table rows below a gather workgroup's targets held in LDS
 
 template <int KT>
 struct DetLayout {
@@ -225,6 +226,31 @@ __device__ __forceinline__ double det_rate(const double* tab, int j, int k)
     return j == k ? 0.5 * pr : pr;
 }
 
+// Table reads of the gather: element c of bin i's row, from the workgroup's
+// LDS copy of the rows [wlo, whi) (LDS = true; every row its pairs touch lies
+// there) or from global memory.
+template <int TS, bool LDS>
+struct DetTabWin {
+    const double* ts;    // LDS copy of tab rows [wlo, whi)
+    const double* tab;
+    int wlo;
+    __device__ __forceinline__ double operator()(int i, int c) const
+    {
+        if constexpr (LDS) return lds_ld(&ts[(i - wlo) * TS + c]);
+        else return glb_ld(&tab[(size_t)TS * i + c]);
+    }
+};
+
+template <int KT, class TW>
+__device__ __forceinline__ double det_rate_w(const TW& T, int j, int k)
+{
+    constexpr int R = DetLayout<KT>::R;
+    double pr = T(j, 0) * T(k, R);
+#pragma unroll
+    for (int r = 1; r < R; ++r) pr = fma(T(j, r), T(k, R + r), pr);
+    return j == k ? 0.5 * pr : pr;
+}
+
 // Pivot reads of the gather: from the workgroup's LDS copy of x[wlo, whi)
 // (LDS = true; every index its pairs touch lies there) or from global memory.
 template <bool LDS>
@@ -244,14 +270,13 @@ struct DetXWin {
 // Columns k = Lb - 1 - sub, Lb - 1 - sub - kDetSplit, ... (this lane's share
 // of the target's columns, walked downward): the rows of a column whose sum
 // lands in the bracket form a range [ja, jb) that only moves up as k falls.
-template <int KT, class XW>
-__device__ __forceinline__ double det_near(const DetAggArgs& a, const XW& X, int Lb,
-                                           bool upper, int sub)
+template <int KT, class XW, class TW>
+__device__ __forceinline__ double det_near(const XW& X, const TW& T, int Lb, bool upper,
+                                           int sub)
 {
-    using L = DetLayout<KT>;
-    constexpr int R = L::R, TS = L::TS;
+    constexpr int R = DetLayout<KT>::R;
     const double xL = X(Lb), xL1 = X(Lb + 1);
-    const double idL = a.tab[(size_t)TS * Lb + 2 * R + 1];
+    const double idL = T(Lb, 2 * R + 1);
     double acc = 0.0;
     int k = Lb - 1 - sub;
     if (k < 0 || X(k) + X(k) < xL) return 0.0;   // nor can any smaller column
@@ -265,7 +290,7 @@ __device__ __forceinline__ double det_near(const DetAggArgs& a, const XW& X, int
         while (jb <= k && X(jb) + xk < xL1) ++jb;
         const int je = min(jb, k + 1);   // rows j <= k
         for (int j = ja; j < je; ++j) {
-            const double pr = det_rate<KT>(a.tab, j, k);
+            const double pr = det_rate_w<KT>(T, j, k);
             const double w = (X(j) + xk - xL) * idL;
             acc += upper ? w * pr : pr - w * pr;
         }
@@ -279,14 +304,19 @@ __global__ __launch_bounds__(kDetGather) void det_agg_gather_kernel(DetAggArgs a
     using L = DetLayout<KT>;
     constexpr int R = L::R, TS = L::TS;
     __shared__ double xs[kDetXBelow + kDetTargets + 1];
+    __shared__ double ts[(kDetTabBelow + kDetTargets + 1) * TS];
     const int n = a.n, S = a.S;
     const int tid = threadIdx.x;
     // pivots of the neighbourhood: the targets' brackets and the rows and
     // columns of their near pairs (on a grid of ratio 1.01 or more, all of
-    // them); the barriers below order these stores before the reads
+    // them), and the table rows of the nearer part (ratio ~1.015 or more);
+    // the barrier below orders these stores before the reads
     const int i0 = blockIdx.x * kDetTargets;
     const int wlo = max(0, i0 - kDetXBelow), whi = min(n, i0 + kDetTargets + 1);
+    const int twlo = max(0, i0 - kDetTabBelow);
     for (int t = tid; t < whi - wlo; t += kDetGather) xs[t] = a.x[wlo + t];
+    for (int t = tid; t < (whi - twlo) * TS; t += kDetGather)
+        ts[t] = a.tab[(size_t)TS * twlo + t];
     // TB_r = sum of all column factors in a fixed order (each lane its
     // strided block totals in turn, then the wave's butterfly), the same in
     // every lane and every workgroup
@@ -309,46 +339,59 @@ __global__ __launch_bounds__(kDetGather) void det_agg_gather_kernel(DetAggArgs a
     // row of a pair reaching x_L, from column L-1 where the rows reach
     // lowest, lies above wlo: x_wlo + x_{L-1} < x_L.  Columns and rows then
     // stay in [wlo, L].
-    bool inwin = true;
-    if (wlo > 0) {
+    // The same test against the first table row held in LDS says whether
+    // the table reads can come from there too (then so can the pivots).
+    bool inwin = true, intab = true;
+    {
         // lane t <= kDetTargets checks L = i0 - 1 + t: the pivots [i0 - 1, i0 + 15]
         const int L = min(i0 - 1 + min(tid, kDetTargets), n - 1);
-        inwin = xs[0] + a.x[L - 1] < a.x[L];
-        if (i0 + kDetTargets >= n - 1) inwin = inwin && xs[0] + a.x[n - 2] < a.x[n - 1];
+        const bool clipwg = i0 + kDetTargets >= n - 1;
+        if (wlo > 0) {
+            inwin = xs[0] + a.x[L - 1] < a.x[L];
+            if (clipwg) inwin = inwin && xs[0] + a.x[n - 2] < a.x[n - 1];
+        }
+        if (twlo > 0) {
+            const double xt = xs[twlo - wlo];
+            intab = xt + a.x[L - 1] < a.x[L];
+            if (clipwg) intab = intab && xt + a.x[n - 2] < a.x[n - 1];
+        }
     }
     inwin = __all(inwin);
+    intab = __all(intab) && inwin;
     const DetXWin<true> XL{xs, a.x, wlo, whi};
     const DetXWin<false> XG{xs, a.x, 0, n};
+    const DetTabWin<TS, true> TL{ts, a.tab, twlo};
+    const DetTabWin<TS, false> TG{ts, a.tab, 0};
 
     // clipped pairs (v >= x_{n-1}, k <= n-2): the workgroup that owns bin n-1
     double clip = 0.0;
     if (i0 <= n - 1 && n - 1 < i0 + kDetTargets && n >= 2) {
-        auto clipped = [&](const auto& X) {
+        auto clipped = [&](const auto& X, const auto& T) {
             const double xl = X(n - 1);
             double c = 0.0;
             for (int k = n - 2 - tid; k >= X.wlo; k -= kDetGather) {
                 const double xk = X(k);
                 if (xk + xk < xl) break;   // smaller k cannot reach x_{n-1}
                 const int jc = first_true(X.wlo, k + 1, [&](int j) { return X(j) + xk >= xl; });
-                for (int j = jc; j <= k; ++j) c += det_rate<KT>(a.tab, j, k);
+                for (int j = jc; j <= k; ++j) c += det_rate_w<KT>(T, j, k);
             }
             return c;
         };
-        clip = wave_sum(inwin ? clipped(XL) : clipped(XG));
+        clip = wave_sum(intab ? clipped(XL, TL) : inwin ? clipped(XL, TG) : clipped(XG, TG));
     }
 
     // near births: each lane its share of the target's columns, the four
     // shares summed in a fixed order ((s0 + s1) + (s2 + s3), the same value
     // in all four lanes)
     double nl = 0.0, nu = 0.0;
+    auto near = [&](const auto& X, const auto& T) {
+        if (i <= n - 2) nl = det_near<KT>(X, T, i, false, sub);
+        if (i >= 1 && i - 1 <= n - 2) nu = det_near<KT>(X, T, i - 1, true, sub);
+    };
     if (i < n) {
-        if (inwin) {
-            if (i <= n - 2) nl = det_near<KT>(a, XL, i, false, sub);
-            if (i >= 1 && i - 1 <= n - 2) nu = det_near<KT>(a, XL, i - 1, true, sub);
-        } else {
-            if (i <= n - 2) nl = det_near<KT>(a, XG, i, false, sub);
-            if (i >= 1 && i - 1 <= n - 2) nu = det_near<KT>(a, XG, i - 1, true, sub);
-        }
+        if (intab) near(XL, TL);
+        else if (inwin) near(XL, TG);
+        else near(XG, TG);
     }
     nl += dpp_move<0xB1, 0xF>(nl);   // quad_perm 1,0,3,2
     nl += dpp_move<0x4E, 0xF>(nl);   // quad_perm 2,3,0,1
