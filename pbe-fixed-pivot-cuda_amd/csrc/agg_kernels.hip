@@ -196,6 +196,69 @@ void agg_strip_kernel(AggArgs a, StripPlan plan)
     }
 }
 
+// Mid-size problems: the band sweep items and strip shares of four waves in
+// one launch, side by side (two 256-thread workgroups per CU), so that the
+// two latency chains run together instead of one after the other.  The sweep
+// items come first in the grid; the strip shares are the rest, all resident
+// at once.
+constexpr int kFusedWaves = 4;
+constexpr int kFusedMaxBlocks = 32;   // 256-column blocks (n <= 8192)
+
+template <int KT>
+union FusedSmem {
+    BandSmem<KT> band;
+    StripSmem<KT, kBandT / kWave, kFusedWaves> strip;
+};
+
+template <int KT>
+__global__ __launch_bounds__(kWave * kFusedWaves, 2)
+void agg_fused_kernel(AggArgs a, int nseg, BandSegs segs, StripPlan plan, int nsweep)
+{
+    constexpr int U = kBandT / kWave;
+    __shared__ FusedSmem<KT> sm;
+    if ((int)blockIdx.x < nsweep) {
+        agg_band_sweep<KT>(a, blockIdx.x / nseg, blockIdx.x % nseg, segs, sm.band);
+        return;
+    }
+    const long long CT = strip_cost_total(plan), G = gridDim.x - nsweep, g = blockIdx.x - nsweep;
+    int qa, ca, qb, cb;
+    strip_first_unit(plan, strip_chunk_at_cost(plan, CT * g / G), qa, ca);
+    strip_first_unit(plan, strip_chunk_at_cost(plan, CT * (g + 1) / G), qb, cb);
+    if (qa > qb || (qa == qb && ca >= cb)) return;   // empty share
+    const int qe = cb > 0 ? qb + 1 : qb;
+    StripCols<KT, U, kFusedWaves> cols;
+    cols.load(a, qa);
+    for (int i = threadIdx.x; i < StripSmem<KT, U, kFusedWaves>::WIN; i += blockDim.x)
+        sm.strip.colwin[i] = 0.0;
+    for (int q = qa, c = ca; q < qe; ++q, c = 0) {
+        const int hi = (q == qb) ? cb : strip_nc(plan, q);
+        agg_strip_visit<KT, U, kFusedWaves>(a, q, c, hi, q + 1 < qe ? q + 1 : -1, cols,
+                                             sm.strip);
+    }
+}
+
+// Workgroups of the fused kernel resident on the current device at once.
+template <int KT>
+static int fused_resident_groups()
+{
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    int v = cache[dev].load(std::memory_order_relaxed);
+    if (v > 0) return v;
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(&agg_fused_kernel<KT>), kWave * kFusedWaves,
+            0) != hipSuccess || per_cu <= 0)
+        per_cu = 1;
+    v = cus * per_cu;
+    cache[dev].store(v, std::memory_order_relaxed);
+    return v;
+}
+
 // Workgroups of the strip kernel resident on the current device at once.
 template <int KT, int U>
 static int resident_groups()
@@ -240,6 +303,19 @@ static hipError_t launch_agg_t(AggArgs args, int split, hipStream_t stream)
         return hipSuccess;
     }
     if (band_items > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (T == kBandT && args.nb <= kFusedMaxBlocks) {
+        // sweep items on at most half of the resident slots, strips on the rest
+        const long long Gf = fused_resident_groups<KT>();
+        int Sf = split;
+        while (Sf > 1 && (long long)args.nb * Sf * 2 > Gf) Sf /= 2;
+        const long long nsweep = (long long)args.nb * Sf;
+        if (nsweep * 2 <= Gf) {
+            hipLaunchKernelGGL((agg_fused_kernel<KT>), dim3((unsigned)Gf),
+                               dim3(kWave * kFusedWaves), 0, stream, args, Sf,
+                               band_segments_cached(Sf, args.inv_log_r), plan, (int)nsweep);
+            return hipSuccess;
+        }
+    }
     if (T == kBandT) {
         // band as diagonal sweeps: split = offset segments per block
         hipLaunchKernelGGL((agg_sweep_kernel<KT>), dim3((unsigned)(args.nb * split)), dim3(256),

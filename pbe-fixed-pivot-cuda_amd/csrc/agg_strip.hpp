@@ -123,13 +123,13 @@ __device__ __forceinline__ void wave_lds_sync()
     __asm__ __volatile__("" ::: "memory");
 }
 
-template <int KT, int U>
+template <int KT, int U, int NW = StripWaves<U>::value>
 struct StripSmem {
     static constexpr int R = Kern<KT>::R;
     static constexpr int RS = (R + 2) & ~1;
     static constexpr int T = kWave * U;
     static constexpr int WIN = T + kStripExtra;
-    static constexpr int NWV = StripWaves<U>::value;
+    static constexpr int NWV = NW;
     double rowbuf[NWV][(kStripPieceRows + 2) * RS];   // per wave: row factors A_r(j) and x_j
     double colwin[WIN];                  // births/deaths of the window bins
     double xw[WIN + 1];                  // pivots of the window bins
@@ -141,8 +141,8 @@ struct StripSmem {
 };
 
 // Column factors B_r(k) = N_k phi_r(k) of this lane's U columns.
-template <int KT, int U>
-__device__ __forceinline__ void strip_col_factors(const AggArgs& a, const StripSmem<KT, U>& sm,
+template <int KT, int U, int NW>
+__device__ __forceinline__ void strip_col_factors(const AggArgs& a, const StripSmem<KT, U, NW>& sm,
                                                   const int k0, const int lane,
                                                   double (&Bc)[U][Kern<KT>::R])
 {
@@ -166,11 +166,11 @@ __device__ __forceinline__ void strip_col_factors(const AggArgs& a, const StripS
 
 // Column data of block q held in registers between visits: window pivots,
 // populations and right neighbours of the block's columns.
-template <int KT, int U>
+template <int KT, int U, int NWAVES = StripWaves<U>::value>
 struct StripCols {
-    static constexpr int T = StripSmem<KT, U>::T;
-    static constexpr int WIN = StripSmem<KT, U>::WIN;
-    static constexpr int NWV = StripSmem<KT, U>::NWV;
+    static constexpr int T = StripSmem<KT, U, NWAVES>::T;
+    static constexpr int WIN = StripSmem<KT, U, NWAVES>::WIN;
+    static constexpr int NWV = StripSmem<KT, U, NWAVES>::NWV;
     static constexpr int NTH = kWave * NWV;   // threads of the workgroup
     static constexpr int NW = (WIN + 1 + NTH - 1) / NTH;
     static constexpr int NT = (T + NTH - 1) / NTH;
@@ -214,17 +214,18 @@ struct StripCols {
 // `cols` holds block next_q's (if next_q >= 0, loaded before the window's
 // global atomics so those loads do not wait behind them) and this thread's
 // window bins are flushed and zero again.
-template <int KT, int U>
+template <int KT, int U, int NW = StripWaves<U>::value>
 __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, const int c_lo,
                                                 const int c_hi, const int next_q,
-                                                StripCols<KT, U>& cols, StripSmem<KT, U>& sm)
+                                                StripCols<KT, U, NW>& cols,
+                                                StripSmem<KT, U, NW>& sm)
 {
     using K = Kern<KT>;
-    using Cols = StripCols<KT, U>;
+    using Cols = StripCols<KT, U, NW>;
     constexpr int R = K::R;
-    constexpr int RS = StripSmem<KT, U>::RS;
-    constexpr int T = StripSmem<KT, U>::T;
-    constexpr int WIN = StripSmem<KT, U>::WIN;
+    constexpr int RS = StripSmem<KT, U, NW>::RS;
+    constexpr int T = StripSmem<KT, U, NW>::T;
+    constexpr int WIN = StripSmem<KT, U, NW>::WIN;
     constexpr int NTH = Cols::NTH;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
     const int n = a.n;
@@ -262,7 +263,7 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
     double TK[R];
     {
         double Bc[U][R];
-        strip_col_factors<KT, U>(a, sm, k0, lane, Bc);
+        strip_col_factors<KT, U, NW>(a, sm, k0, lane, Bc);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             double s = 0.0;
@@ -451,7 +452,7 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
                                 : 1.0 / (glb_ld(&a.x[i + 1]) - glb_ld(&a.x[i])));
             };
             double Bc[U][R];
-            strip_col_factors<KT, U>(a, sm, k0, lane, Bc);
+            strip_col_factors<KT, U, NW>(a, sm, k0, lane, Bc);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int k = k0 + lane + kWave * u;
@@ -503,7 +504,7 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
     // rate (1 - w) to bin k, rate w to bin k+1 and takes rate from bin k, so
     // bin k nets -rate w and bin k+1 +rate w ----
     double Bc[U][R];
-    strip_col_factors<KT, U>(a, sm, k0, lane, Bc);
+    strip_col_factors<KT, U, NW>(a, sm, k0, lane, Bc);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int c = lane + kWave * u;

@@ -267,6 +267,34 @@ def test_strips_fine_grid(pbe, oracle_mod, r, kernel):
 
 
 @pytest.mark.parametrize("kernel", range(6))
+def test_fused_sweep_and_strips(pbe, oracle_mod, kernel):
+    """Mid sizes (at most 32 blocks of 256 columns): the band sweep items and
+    the four-wave strip shares in one launch, for every band split (the
+    launch halves it while the sweep would take over half the slots), on a
+    geometric grid (start-bracket offsets) and a jittered one (searches)."""
+    from pbe_hip import hip, problems as P
+    for n, jitter in ((3400, 0.0), (5000, 0.0), (8192, 0.0), (6000, 0.2)):
+        x = (P.geometric_grid(n, 1e-18, 1.02) if jitter == 0.0
+             else P.jittered_grid(n, 1e-18, 1.02, jitter, seed=n))
+        N = P.lognormal_ic(x, x[n // 3], 1.5, 1e12, noise=0.05, seed=60 + kernel)
+        log_x0, inv_log_r = P.grid_lookup(x) if jitter == 0.0 else (0.0, 0.0)
+        agg = dict(kernel=kernel, beta0=1e-17 if kernel != 2 else 1e-17 / x[n // 3],
+                   beta_br=6.73e-18, beta_sh=1e-3, log_x0=log_x0, inv_log_r=inv_log_r)
+        ref, A, D = _oracle(oracle_mod, N, x, agg=agg)
+        dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
+        drhs = hip.DeviceArray(n)
+        p = pbe.agg_params(n, agg)
+        for split in (1, 4, 16):
+            drhs.zero()
+            hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=4, band_split=split))
+            hip.synchronize()
+            got = drhs.download()
+            what = f"fused kernel {kernel} n={n} jitter {jitter} split {split}"
+            assert_parity(got, ref, A, what=what)
+            assert_m1(got, x, A, D, what=what)
+
+
+@pytest.mark.parametrize("kernel", range(6))
 def test_band_every_split(pbe, oracle_mod, kernel):
     """n = 9000: the column strips plus the diagonal band, for every band
     split and the 128-, 256- and 512-column blocks."""
