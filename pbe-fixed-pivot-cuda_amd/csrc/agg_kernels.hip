@@ -292,19 +292,9 @@ static hipError_t launch_agg_t(AggArgs args, int split, hipStream_t stream)
     plan.U = U;
     plan.nb = args.nb;
     plan.total = strip_total(plan);
-    const long long G = resident_groups<KT, U>();
-    if (plan.total < G) {
-        // too little far work for the strips: one launch of all tiles
-        const long long tiles = (long long)args.nb * (args.nb + 1) / 2;
-        const long long items = band_items + tiles - band_tiles(args.nb);
-        if (items > 0x7fffffffLL) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((agg_tiles_kernel<KT, U>), dim3((unsigned)items), dim3(256), 0,
-                           stream, args, split);
-        return hipSuccess;
-    }
-    if (band_items > 0x7fffffffLL) return hipErrorInvalidValue;
-    if (T == kBandT && args.nb <= kFusedMaxBlocks) {
-        // sweep items on at most half of the resident slots, strips on the rest
+    if (T == kBandT && args.nb <= kFusedMaxBlocks && plan.total > 0) {
+        // mid sizes: sweep items on at most half of the resident slots of
+        // the fused kernel, strip shares on the rest (empty shares return)
         const long long Gf = fused_resident_groups<KT>();
         int Sf = split;
         while (Sf > 1 && (long long)args.nb * Sf * 2 > Gf) Sf /= 2;
@@ -316,6 +306,17 @@ static hipError_t launch_agg_t(AggArgs args, int split, hipStream_t stream)
             return hipSuccess;
         }
     }
+    const long long G = resident_groups<KT, U>();
+    if (plan.total < G) {
+        // too little far work for the strips: one launch of all tiles
+        const long long tiles = (long long)args.nb * (args.nb + 1) / 2;
+        const long long items = band_items + tiles - band_tiles(args.nb);
+        if (items > 0x7fffffffLL) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((agg_tiles_kernel<KT, U>), dim3((unsigned)items), dim3(256), 0,
+                           stream, args, split);
+        return hipSuccess;
+    }
+    if (band_items > 0x7fffffffLL) return hipErrorInvalidValue;
     if (T == kBandT) {
         // band as diagonal sweeps: split = offset segments per block
         hipLaunchKernelGGL((agg_sweep_kernel<KT>), dim3((unsigned)(args.nb * split)), dim3(256),
