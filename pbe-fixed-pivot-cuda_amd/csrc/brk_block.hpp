@@ -69,7 +69,7 @@ __device__ __forceinline__ void brk_block_body(const BrkArgs& a, const int p0, B
         const int i = tid + 256 * t;
         const int j = p0 + i;
         double s = 0.0;
-        if (j < n) {
+        if (i < P && j < n) {
             const double xj = xw[j - wbase];
             const double Nj = nj[t];
             const double Sj = selection<SEL>(xj, a.S0, a.v_ref, a.alpha, a.v_min);
@@ -78,7 +78,7 @@ __device__ __forceinline__ void brk_block_body(const BrkArgs& a, const int p0, B
                 put(j, -s);
             }
         }
-        sn[i] = s;
+        if (i < P) sn[i] = s;
     }
     __syncthreads();
 
