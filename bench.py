@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the batched and adjoint comparisons")
     ap.add_argument("--no-variants", action="store_true",
-                    help="skip the n = 1024 / 4096 / GL16 variants (profiling runs)")
+                    help="skip the n = 1024 / 4096 / GL16 / 65536 variants (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run an elementwise device copy of BYTES (torch.neg, "
                          "reads and writes BYTES each), for scaling HBM byte counters")
@@ -142,10 +142,10 @@ def adjoint_form(pbe, hip, P):
     return out
 
 
-def rhs_variant(pbe_hip, hip, P, n, daughter, steps=20):
+def rhs_variant(pbe_hip, hip, P, n, daughter, steps=20, r=1.02):
     """Average step time (zero rhs, aggregation, breakage) of one flocculation
     RHS configuration, HIP events on the launch stream."""
-    x, N, agg, brk, t_q, bw_q = P.config_flocculation(n, daughter=daughter, n_quad=16)
+    x, N, agg, brk, t_q, bw_q = P.config_flocculation(n, daughter=daughter, n_quad=16, r=r)
     nq = len(t_q)
     dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
     dt, db = hip.DeviceArray.from_host(t_q), hip.DeviceArray.from_host(bw_q)
@@ -168,7 +168,7 @@ def rhs_variant(pbe_hip, hip, P, n, daughter, steps=20):
     e1.record(s)
     s.synchronize()
     ms = e0.elapsed_ms(e1) / steps
-    return {"n": n, "daughter": daughter, "n_quad": nq, "ms_per_step": ms,
+    return {"n": n, "r": r, "daughter": daughter, "n_quad": nq, "ms_per_step": ms,
             "pair_updates_per_s": P.pair_units(n, nq) / (ms * 1e-3)}
 
 
@@ -372,6 +372,9 @@ def main():
     if not args.no_variants:
         out["variants"] = [rhs_variant(pbe_hip, hip, P, m, "symmetric") for m in (1024, 4096)]
         out["variants"].append(rhs_variant(pbe_hip, hip, P, n, "uniform"))
+        # past the int32 pair count (2.1e9 pairs); r = 1.005 keeps x finite
+        out["variants"].append(rhs_variant(pbe_hip, hip, P, 65536, "symmetric", steps=5,
+                                           r=1.005))
     if not args.no_forms:
         out["forms"] = {"batched": batched_form(hip, P), "adjoint": adjoint_form(pbe_hip, hip, P),
                         "deterministic": deterministic_form(pbe_hip, hip, P, n)}

@@ -4,6 +4,7 @@
 // src/breakage.cu:23-24,46,74 of the reference (n, then pointers, then the
 // enum).  Every check runs before the first HIP runtime call.
 #include <algorithm>
+#include <cmath>
 
 #include "pbe_internal.hpp"
 
@@ -47,6 +48,24 @@ int default_tile_u(int n, int kernel_type) { return tile_rule(n, kernel_type).u;
 
 int default_band_split(int n, int kernel_type) { return tile_rule(n, kernel_type).w; }
 
+// The table's columns per lane, widened on fine geometric grids.  A strip
+// pair (j, k) lies more than T = 64 U bins off the diagonal, and it is far
+// (both births in bins k, k+1) when x_j < (r - 1) x_k, i.e. k - j > log_r
+// (1 / (r - 1)) bins; a strip chunk holding a nearer pair takes the per-pair
+// path for all its pairs, and those chunks sit together in one or two
+// workgroups.  So past the fused mid-size launch (n > 8192, T = 256 there),
+// a grid whose reach exceeds T but not 512 bins takes U = 8: measured at
+// n = 16384, r = 1.01 (reach 463 bins) 0.12 ms against 0.40 ms at U = 4.
+// Finer grids (reach > 512) keep the table's choice, as do general grids.
+int grid_tile_u(const pbe_aggregation_params& p)
+{
+    const int u = default_tile_u(p.n, p.kernel_type);
+    const double il = p.inv_log_r;
+    if (p.n <= 8192 || !(il > 0.0) || !std::isfinite(il) || u == 8) return u;
+    const double reach = -std::log(std::expm1(1.0 / il)) * il;   // log_r (1 / (r - 1))
+    return (reach > 64.0 * u && reach <= 512.0) ? 8 : u;
+}
+
 }  // namespace pbe
 
 extern "C" hipError_t pbe_launch_aggregation_rhs(const double* N, const double* x,
@@ -59,7 +78,7 @@ extern "C" hipError_t pbe_launch_aggregation_rhs(const double* N, const double* 
     if (p->kernel_type < 0 || p->kernel_type > 5) return hipErrorInvalidValue;
     if (p->block_size > 1024) return hipErrorInvalidConfiguration;
     hipError_t e = pbe::launch_aggregation(N, x, rhs, *p,
-                                           pbe::default_tile_u(p->n, p->kernel_type),
+                                           pbe::grid_tile_u(*p),
                                            pbe::default_band_split(p->n, p->kernel_type),
                                            stream);
     if (e != hipSuccess) return e;
@@ -78,7 +97,7 @@ extern "C" hipError_t pbe_launch_aggregation_rhs_ex(const double* N, const doubl
     if (p->block_size > 1024) return hipErrorInvalidConfiguration;
     int u = opts ? opts->tile_u : 0;
     int w = opts ? opts->band_split : 0;
-    if (u == 0) u = pbe::default_tile_u(p->n, p->kernel_type);
+    if (u == 0) u = pbe::grid_tile_u(*p);
     if (w == 0) w = pbe::default_band_split(p->n, p->kernel_type);
     if (u != 1 && u != 2 && u != 4 && u != 8) return hipErrorInvalidValue;
     if (w != 1 && w != 2 && w != 4 && w != 8 && w != 16) return hipErrorInvalidValue;

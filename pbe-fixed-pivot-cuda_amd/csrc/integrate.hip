@@ -72,7 +72,7 @@ hipError_t enqueue_rhs(const RhsProblem& p, const double* Nin, double* rhs, hipS
     if (p.batch) return enqueue_batched_rhs(p.batch, Nin, p.x, p.t_q, p.bw_q, rhs, s);
     if (p.agg) {
         e = launch_aggregation(Nin, p.x, rhs, *p.agg,
-                               default_tile_u(p.agg->n, p.agg->kernel_type),
+                               grid_tile_u(*p.agg),
                                default_band_split(p.agg->n, p.agg->kernel_type), s);
         if (e != hipSuccess) return e;
     }

@@ -19,6 +19,8 @@ hipError_t launch_breakage(const double* N, const double* x, const double* t_q,
 
 // Columns per lane picked for an aggregation problem of size n.
 int default_tile_u(int n, int kernel_type);
+// The same, widened to 8 on a fine geometric grid (capi.hip).
+int grid_tile_u(const pbe_aggregation_params& p);
 
 // Atomic-free, bitwise repeatable forms (det_kernels.hip); ws holds
 // det_*_workspace_doubles(n) doubles.

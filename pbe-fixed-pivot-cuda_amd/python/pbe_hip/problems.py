@@ -156,13 +156,13 @@ def config_smoluchowski(n=256):
 
 
 def config_flocculation(n, seed=20261203, daughter="symmetric", n_quad=16,
-                        G=100.0):
-    """Single-instance aggregation + breakage: geometric grid r = 1.02 from
-    x_0 = 1e-18, Brownian+shear kernel beta_br = 6.73e-18, beta_sh = 4/3 G,
+                        G=100.0, r=1.02):
+    """Single-instance aggregation + breakage: geometric grid r = 1.02 (or
+    the ratio given; n ln r must stay below about 700) from x_0 = 1e-18, Brownian+shear kernel beta_br = 6.73e-18, beta_sh = 4/3 G,
     threshold breakage S0 = 1e-3 with v_min = x_{n/2}, symmetric-binary
     daughters (or a GL uniform variant), lognormal N (median x_{n/4},
     sigma_ln = 0.8, total 1e12, +-5 % seeded noise)."""
-    x = geometric_grid(n, 1e-18, 1.02)
+    x = geometric_grid(n, 1e-18, r)
     N = lognormal_ic(x, x[n // 4], 0.8, 1e12, noise=0.05, seed=seed)
     log_x0, inv_log_r = grid_lookup(x)
     agg = dict(kernel=5, beta0=0.0, beta_br=6.73e-18, beta_sh=4.0 / 3.0 * G,

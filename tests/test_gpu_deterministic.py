@@ -173,6 +173,37 @@ def test_det_flocculation_16384(pbe, oracle_mod):
     print(f"det n=16384: entry {w:.2e}")
 
 
+def test_det_beyond_int32_pair_count(pbe, oracle_mod):
+    """n = 65536 (2.1e9 pairs, past the reference's n = 23,169 int limit):
+    bitwise repeatability, and every entry against the OpenMP port with the
+    first moment, aggregation and breakage apart, as for the atomic launches
+    at this size."""
+    from pbe_hip import problems as P
+    n = 65536
+    x = P.geometric_grid(n, 1e-18, 1.005)
+    N = P.lognormal_ic(x, x[n // 4], 0.8, 1e12, noise=0.05, seed=n)
+    log_x0, inv_log_r = P.grid_lookup(x)
+    agg = dict(kernel=5, beta0=0.0, beta_br=6.73e-18, beta_sh=4.0 / 3.0 * 100.0,
+               log_x0=log_x0, inv_log_r=inv_log_r)
+    brk = dict(selection=3, S0=1e-3, v_ref=1.0, alpha=1.0, v_min=x[n // 4])
+    t_q, bw_q = P.daughter_symmetric()
+    for what, kw, port in (
+            ("aggregation", dict(agg=agg),
+             lambda: oracle_mod.port_aggregation_rhs(N, x, threads=16, **agg)),
+            ("breakage", dict(brk=brk, t_q=t_q, bw_q=bw_q),
+             lambda: oracle_mod.port_breakage_rhs(N, x, t_q, bw_q, threads=16, **brk))):
+        runs = [_det(pbe, N, x, **kw) for _ in range(2)]
+        assert np.array_equal(runs[0].view(np.uint64), runs[1].view(np.uint64)), \
+            f"det {what} n={n}: repeated calls differ"
+        got, p = runs[0], port()
+        scale = np.abs(p).max()
+        err = np.abs(got - p) / (1e-10 * scale + 1e-9 * np.abs(p))
+        assert err.max() <= 1.0, f"det {what} n={n}: bin {int(np.argmax(err))} off the port"
+        m1 = abs(float(np.sum(x * got))) / float(np.sum(x * np.abs(p)))
+        assert m1 <= 1e-13, f"det {what} n={n}: |sum x rhs| / sum x |port| = {m1:.3e}"
+        print(f"det {what} n={n}: m1 {m1:.2e}")
+
+
 def test_det_workspace_checks(pbe):
     """A missing or short workspace is refused before any launch."""
     from pbe_hip import hip, problems as P

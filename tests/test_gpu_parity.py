@@ -189,6 +189,51 @@ def test_flocculation_16384_vs_port(pbe, oracle_mod):
     print(f"n=16384: entry {w:.2e} m1 {m:.2e}")
 
 
+def _vs_port(got, port, x, what):
+    """Every entry against the OpenMP port, as for n = 16384, and the first
+    moment: sum x got within 1e-13 of sum x |port| (sum x A is 3.2 times
+    larger at n = 32768, so this is the stricter bound)."""
+    scale = np.abs(port).max()
+    err = np.abs(got - port) / (1e-10 * scale + 1e-9 * np.abs(port))
+    assert err.max() <= 1.0, f"{what}: bin {int(np.argmax(err))} off the port"
+    m1 = abs(float(np.sum(x * got))) / float(np.sum(x * np.abs(port)))
+    assert m1 <= 1e-13, f"{what}: |sum x rhs| / sum x |port| = {m1:.3e}"
+    return m1
+
+
+@pytest.mark.parametrize("n,r,search,exact", [(32768, 1.01, False, True),
+                                              (40007, 1.008, True, False),
+                                              (65536, 1.005, False, False)])
+def test_beyond_int32_pair_count(pbe, oracle_mod, n, r, search, exact):
+    """Sizes past the reference's n = 23,169 (its triangle decode computes
+    (2n+1)^2 in int, src/aggregation_kernels.cuh:171): 5.4e8 to 2.1e9 pairs
+    through the band sweep and the column strips (64-bit plan).  Geometric
+    grids with the log-ratio lookup, and a ragged n (157 column blocks, the
+    last partial) with the binary search; aggregation and breakage checked
+    apart (their magnitudes differ by 35 decades here), at n = 32768 also
+    against the exact oracle."""
+    from pbe_hip import problems as P
+    x = P.geometric_grid(n, 1e-18, r)
+    N = P.lognormal_ic(x, x[n // 4], 0.8, 1e12, noise=0.05, seed=n)
+    log_x0, inv_log_r = (0.0, 0.0) if search else P.grid_lookup(x)
+    agg = dict(kernel=5, beta0=0.0, beta_br=6.73e-18, beta_sh=4.0 / 3.0 * 100.0,
+               log_x0=log_x0, inv_log_r=inv_log_r)
+    brk = dict(selection=3, S0=1e-3, v_ref=1.0, alpha=1.0, v_min=x[n // 4])
+    t_q, bw_q = P.daughter_symmetric()
+    ga = _gpu(pbe, N, x, agg=agg)
+    gb = _gpu(pbe, N, x, brk=brk, t_q=t_q, bw_q=bw_q)
+    ma = _vs_port(ga, oracle_mod.port_aggregation_rhs(N, x, threads=16, **agg), x,
+                  f"aggregation n={n}")
+    mb = _vs_port(gb, oracle_mod.port_breakage_rhs(N, x, t_q, bw_q, threads=16, **brk), x,
+                  f"breakage n={n}")
+    print(f"n={n}: m1 aggregation {ma:.2e} breakage {mb:.2e}")
+    if exact:
+        ref, A, D = _oracle(oracle_mod, N, x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+        w = assert_parity(ga + gb, ref, A, what=f"n={n}")
+        m = assert_m1(ga + gb, x, A, D, what=f"n={n}")
+        print(f"n={n} exact oracle: entry {w:.2e} m1 {m:.2e}")
+
+
 @pytest.mark.parametrize("u", [1, 2, 4, 8])
 @pytest.mark.parametrize("n,kernel", [(700, 5), (3000, 5), (3000, 0), (1200, 2)])
 def test_every_tile_shape(pbe, oracle_mod, u, n, kernel):
