@@ -93,7 +93,9 @@ hipError_t pbe_launch_aggregation_rhs(const double* N, const double* x,
  * 30-33).  tile_u = columns per lane: column blocks and diagonal-band tiles
  * are 64*tile_u wide; band_split = row slices per diagonal-band tile (each
  * slice one workgroup).  0 picks the library's autotuned choice for
- * (n, kernel_type). */
+ * (n, kernel_type); on a geometric grid (inv_log_r > 0) with n > 8192 whose
+ * far-pair reach log_r(1/(r-1)) bins exceeds 64*tile_u but not 512, tile_u
+ * = 0 takes 8, so that the column strips hold far pairs only. */
 typedef struct pbe_agg_options {
     int tile_u;       /* 0, 1, 2, 4 or 8 */
     int band_split;   /* 0, 1, 2, 4, 8 or 16 */
@@ -106,7 +108,8 @@ hipError_t pbe_launch_aggregation_rhs_ex(const double* N, const double* x,
                                          const pbe_agg_options* options,
                                          hipStream_t stream);
 
-/* The tile_u that tile_u = 0 resolves to for (n, kernel_type). */
+/* The autotuned tile_u for (n, kernel_type): what tile_u = 0 resolves to,
+ * save the widening to 8 on fine geometric grids described above. */
 int pbe_aggregation_default_tile(int n, int kernel_type);
 
 /*
