@@ -93,9 +93,10 @@ hipError_t pbe_launch_aggregation_rhs(const double* N, const double* x,
  * 30-33).  tile_u = columns per lane: column blocks and diagonal-band tiles
  * are 64*tile_u wide; band_split = row slices per diagonal-band tile (each
  * slice one workgroup).  0 picks the library's autotuned choice for
- * (n, kernel_type); on a geometric grid (inv_log_r > 0) with n > 8192 whose
- * far-pair reach log_r(1/(r-1)) bins exceeds 64*tile_u but not 512, tile_u
- * = 0 takes 8, so that the column strips hold far pairs only. */
+ * (n, kernel_type); on a geometric grid (inv_log_r > 0) whose far-pair
+ * reach log_r(1/(r-1)) bins exceeds 64*tile_u, tile_u = 0 takes 8 when
+ * n >= 6000 and the reach is at most 512 bins (the column strips then hold
+ * far pairs only), or when n >= 32768. */
 typedef struct pbe_agg_options {
     int tile_u;       /* 0, 1, 2, 4 or 8 */
     int band_split;   /* 0, 1, 2, 4, 8 or 16 */

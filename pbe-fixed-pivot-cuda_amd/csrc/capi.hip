@@ -53,17 +53,23 @@ int default_band_split(int n, int kernel_type) { return tile_rule(n, kernel_type
 // (both births in bins k, k+1) when x_j < (r - 1) x_k, i.e. k - j > log_r
 // (1 / (r - 1)) bins; a strip chunk holding a nearer pair takes the per-pair
 // path for all its pairs, and those chunks sit together in one or two
-// workgroups.  So past the fused mid-size launch (n > 8192, T = 256 there),
-// a grid whose reach exceeds T but not 512 bins takes U = 8: measured at
-// n = 16384, r = 1.01 (reach 463 bins) 0.12 ms against 0.40 ms at U = 4.
-// Finer grids (reach > 512) keep the table's choice, as do general grids.
+// workgroups.  So from n = 6000 up, a grid whose reach exceeds T but not 512
+// bins takes U = 8, which leaves the strips far pairs only; a finer grid
+// takes U = 8 from n = 32768 up (fewer, wider near chunks).  Measured, one
+// aggregation call, U = 8 against U = 4: n = 6000, r = 1.01 (reach 463 bins)
+// 0.067 / 0.107 ms; n = 8192, r = 1.01 0.070 / 0.184 ms; n = 16384, r = 1.01
+// 0.12 / 0.40 ms; n = 49152, r = 1.006 (reach 866) 2.58 / 3.81 ms.  Below
+// n = 6000 the fused launch at U = 4 is faster (n = 4096, r = 1.01: 0.061
+// against 0.138 ms), as is U = 4 for the finer grid at n = 16384 (r = 1.005:
+// 0.53 against 0.57 ms).  General grids keep the table's choice.
 int grid_tile_u(const pbe_aggregation_params& p)
 {
     const int u = default_tile_u(p.n, p.kernel_type);
     const double il = p.inv_log_r;
-    if (p.n <= 8192 || !(il > 0.0) || !std::isfinite(il) || u == 8) return u;
+    if (p.n < 6000 || !(il > 0.0) || !std::isfinite(il) || u == 8) return u;
     const double reach = -std::log(std::expm1(1.0 / il)) * il;   // log_r (1 / (r - 1))
-    return (reach > 64.0 * u && reach <= 512.0) ? 8 : u;
+    if (reach <= 64.0 * u) return u;
+    return (reach <= 512.0 || p.n >= 32768) ? 8 : u;
 }
 
 }  // namespace pbe

@@ -339,6 +339,24 @@ def test_fused_sweep_and_strips(pbe, oracle_mod, kernel):
             assert_m1(got, x, A, D, what=what)
 
 
+@pytest.mark.parametrize("n,r,kernel", [(6000, 1.015, 5), (8192, 1.01, 0), (8192, 1.01, 5),
+                                         (12000, 1.015, 3)])
+def test_fine_grid_default_width(pbe, oracle_mod, n, r, kernel):
+    """The default launch on geometric grids whose far-pair reach
+    (282 bins at r = 1.015, 463 at 1.01) exceeds the table's 256-column
+    blocks: the library widens them to 512 columns (no fused launch)."""
+    from pbe_hip import problems as P
+    x = P.geometric_grid(n, 1e-18, r)
+    N = P.lognormal_ic(x, x[n // 3], 1.5, 1e12, noise=0.05, seed=90 + kernel)
+    log_x0, inv_log_r = P.grid_lookup(x)
+    agg = dict(kernel=kernel, beta0=1e-17, beta_br=6.73e-18, beta_sh=1e-3, log_x0=log_x0,
+               inv_log_r=inv_log_r)
+    got = _gpu(pbe, N, x, agg=agg)
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg)
+    assert_parity(got, ref, A, what=f"default width n={n} r={r} kernel {kernel}")
+    assert_m1(got, x, A, D, what=f"default width n={n} r={r} kernel {kernel}")
+
+
 @pytest.mark.parametrize("kernel", range(6))
 def test_band_every_split(pbe, oracle_mod, kernel):
     """n = 9000: the column strips plus the diagonal band, for every band
