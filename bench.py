@@ -206,6 +206,24 @@ def deterministic_form(pbe_hip, hip, P, n=16384, steps=10):
             "pair_updates_per_s": P.pair_units(n, nq) / (ms * 1e-3)}
 
 
+def spot_check(got, N, x, agg, brk, t_q, bw_q):
+    """Every entry of one GPU step against the OpenMP port, within
+    1e-10 max|port| + 1e-9 |port_i| (the port sums in plain FP64 in another
+    order), and the first moment within 1e-13 of sum x |port|."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    port = oracle.port_aggregation_rhs(N, x, threads=threads, **agg)
+    port += oracle.port_breakage_rhs(N, x, t_q, bw_q, threads=threads, **brk)
+    err = np.abs(got - port) / (1e-10 * np.abs(port).max() + 1e-9 * np.abs(port))
+    m1 = abs(float(np.sum(x * got)) - float(np.sum(x * port))) / float(np.sum(x * np.abs(port)))
+    if not (err.max() <= 1.0 and m1 <= 1e-13):
+        raise SystemExit(f"bench spot check failed: entry {err.max():.3e} (of the bound), "
+                         f"first moment {m1:.3e}")
+    return {"vs": "OpenMP port (oracle/pbe_cpu_port.c)", "max_entry_over_bound": float(err.max()),
+            "first_moment_rel": m1}
+
+
 def cpu_baseline(N, x, agg, brk, t_q, bw_q, budget_s):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
@@ -318,10 +336,13 @@ def main():
     stream.synchronize()
     ms_step_nomemset = f0.elapsed_ms(f1) / K
 
-    # correctness spot check of the last evaluated step against the port
+    # correctness spot check of one evaluated step against the OpenMP port
+    # (plain FP64, the reference's bracket rule), outside the timed region;
+    # a mismatch fails the run
     drhs.zero()
     step()
     stream.synchronize()
+    spot = spot_check(drhs.download(), N, x, agg, brk, t_q, bw_q)
 
     flops = P.PAIR_FLOPS[int(agg["kernel"])] * pairs
     achieved = flops / (agg_ms * 1e-3) / 1e12
@@ -352,10 +373,10 @@ def main():
             "step": "hipMemsetAsync(rhs) + aggregation launch + breakage launch",
         },
         "roofline": {
-            # compute-bound; the pipe is FP64 VALU, whose MI355X peak equals
-            # the FP64 matrix peak (78.6 TFLOP/s)
-            "bound": "mfma",
-            "pipe": "fp64 VALU (no matrix instructions: not a contraction)",
+            # compute-bound on the FP64 vector ALU (no matrix instructions:
+            # not a contraction); MI355X's FP64 vector peak is 78.6 TFLOP/s
+            "bound": "valu",
+            "pipe": "fp64 VALU",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -367,6 +388,7 @@ def main():
             "median_launch_ms": agg_median,
         },
         "ms_per_step_without_memset": ms_step_nomemset,
+        "spot_check": spot,
         "wall_s_timed_region": wall,
     }
     if not args.no_variants:

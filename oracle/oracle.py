@@ -42,6 +42,9 @@ def lib():
         L.oracle_aggregation_rhs.restype = i
         L.oracle_aggregation_rhs.argtypes = [i, _dp, _dp, i, d, d, d, d, d, i,
                                              _dp, _dp, _dp]
+        L.oracle_aggregation_rhs_threads.restype = i
+        L.oracle_aggregation_rhs_threads.argtypes = [i, _dp, _dp, i, d, d, d, d, d, i,
+                                                     _dp, _dp, _dp, i]
         L.oracle_breakage_rhs.restype = i
         L.oracle_breakage_rhs.argtypes = [i, i, _dp, _dp, _dp, _dp, i, d, d, d,
                                           d, _dp, _dp, _dp]
@@ -74,21 +77,22 @@ class OracleError(RuntimeError):
 
 
 def aggregation_rhs(N, x, kernel, beta0=1.0, beta_br=0.0, beta_sh=0.0,
-                    log_x0=0.0, inv_log_r=0.0, bracket=BRACKET_EXACT, n=None):
+                    log_x0=0.0, inv_log_r=0.0, bracket=BRACKET_EXACT, n=None, threads=1):
     """Return (rhs, A, m1_defect) of one aggregation evaluation from zero rhs.
 
     A[i] is the sum of |contributions| landing in bin i; m1_defect is the exact
-    sum_i x_i rhs_i produced by boundary clipping.
+    sum_i x_i rhs_i produced by boundary clipping.  threads > 1 splits the rows
+    over OpenMP threads (same sums to double-double precision).
     """
     N, x = _f64(N), _f64(x)
     n = len(N) if n is None else n
     rhs = np.zeros(max(n, 1))
     A = np.zeros(max(n, 1))
     defect = ctypes.c_double(0.0)
-    e = lib().oracle_aggregation_rhs(n, _p(N), _p(x), int(kernel), beta0,
-                                     beta_br, beta_sh, log_x0, inv_log_r,
-                                     int(bracket), _p(rhs), _p(A),
-                                     ctypes.byref(defect))
+    e = lib().oracle_aggregation_rhs_threads(n, _p(N), _p(x), int(kernel), beta0,
+                                             beta_br, beta_sh, log_x0, inv_log_r,
+                                             int(bracket), _p(rhs), _p(A),
+                                             ctypes.byref(defect), int(threads))
     if e:
         raise OracleError(e)
     return rhs, A, defect.value
@@ -272,3 +276,19 @@ def cs_loss(N0, x, dt, steps, logT, mask, eps, agg=None, brk=None, t_q=None, bw_
     if e:
         raise OracleError(e)
     return Lr.value, Li.value / h
+
+
+def cs_rk4_final(N0, x, dt, steps, agg=None, brk=None, t_q=None, bw_q=None, threads=1):
+    """Real RK4 final state through the complex-step code path (imaginary
+    parts zero): the same trajectory cs_loss differentiates."""
+    p, th, keep = _cs_setup(x, agg, brk, t_q, bw_q, threads)
+    n = len(N0)
+    zero = np.zeros(n)
+    Lr, Li = ctypes.c_double(), ctypes.c_double()
+    fre, fim = np.zeros(n), np.zeros(n)
+    e = _cs_fns().cs_rk4_loss(ctypes.byref(p), _p(th), _p(np.zeros(5)), _p(_f64(N0)), _p(zero),
+                              dt, int(steps), _p(zero), _p(zero), 1.0,
+                              ctypes.byref(Lr), ctypes.byref(Li), _p(fre), _p(fim))
+    if e:
+        raise OracleError(e)
+    return fre

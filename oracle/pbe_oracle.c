@@ -124,24 +124,13 @@ static int bin_index_exact(const double* x, int n, double v)
     return hi;
 }
 
-int oracle_aggregation_rhs(int n, const double* N, const double* x,
-                           int kernel, double beta0, double beta_br,
-                           double beta_sh, double log_x0, double inv_log_r,
-                           int bracket_mode, double* rhs, double* abs_sum,
-                           double* m1_defect)
+/* The pairs (j, k >= j) of rows [j_lo, j_hi) into sink s and the clip
+ * defect d: src/aggregation_kernels.cuh:186-200 and :212-231. */
+static void agg_rows(int n, const double* N, const double* x, int kernel, double beta0,
+                     double beta_br, double beta_sh, double log_x0, double inv_log_r,
+                     int bracket_mode, int j_lo, int j_hi, sink_t* s, dd_t* defect)
 {
-    /* validation order of src/aggregation.cu:23-24 and :54 */
-    if (n <= 0) return ORACLE_ERR_INVALID_VALUE;
-    if (!N || !x || !rhs) return ORACLE_ERR_INVALID_POINTER;
-    if (kernel < 0 || kernel > 5) return ORACLE_ERR_INVALID_VALUE;
-
-    sink_t s;
-    s.acc = (dd_t*)calloc((size_t)n, sizeof(dd_t));
-    s.abs = abs_sum;
-    if (!s.acc) return ORACLE_ERR_NO_MEMORY;
-    dd_t defect = {0.0, 0.0};
-
-    for (int j = 0; j < n; ++j) {
+    for (int j = j_lo; j < j_hi; ++j) {
         for (int k = j; k < n; ++k) {
             const int diagonal = (j == k);
             double xj = x[j], xk = x[k], Nj = N[j], Nk = N[k];
@@ -151,34 +140,108 @@ int oracle_aggregation_rhs(int n, const double* N, const double* x,
             if (diagonal) rate *= 0.5;
             /* death, :213-221 */
             if (diagonal) {
-                sink_add(&s, j, -2.0 * rate);
+                sink_add(s, j, -2.0 * rate);
             } else {
-                sink_add(&s, j, -rate);
-                sink_add(&s, k, -rate);
+                sink_add(s, j, -rate);
+                sink_add(s, k, -rate);
             }
             /* birth, :190-200 and :223-231 */
             double v = xj + xk;
             if (v >= x[n - 1]) {
-                sink_add(&s, n - 1, rate);
-                dd_add(&defect, rate * (x[n - 1] - v));
+                sink_add(s, n - 1, rate);
+                dd_add(defect, rate * (x[n - 1] - v));
             } else if (v <= x[0]) {
-                sink_add(&s, 0, rate);
-                dd_add(&defect, rate * (x[0] - v));
+                sink_add(s, 0, rate);
+                dd_add(defect, rate * (x[0] - v));
             } else {
                 int hi = (bracket_mode == ORACLE_BRACKET_REFERENCE)
                          ? bin_index_reference(x, n, v, log_x0, inv_log_r)
                          : bin_index_exact(x, n, v);
                 int lo = hi - 1;
                 double w = (v - x[lo]) / (x[hi] - x[lo]);
-                sink_add(&s, lo, (1.0 - w) * rate);
-                sink_add(&s, hi, w * rate);
+                sink_add(s, lo, (1.0 - w) * rate);
+                sink_add(s, hi, w * rate);
             }
         }
     }
-    for (int i = 0; i < n; ++i) rhs[i] += dd_value(&s.acc[i]);
+}
+
+int oracle_aggregation_rhs_threads(int n, const double* N, const double* x,
+                                   int kernel, double beta0, double beta_br,
+                                   double beta_sh, double log_x0, double inv_log_r,
+                                   int bracket_mode, double* rhs, double* abs_sum,
+                                   double* m1_defect, int threads)
+{
+    /* validation order of src/aggregation.cu:23-24 and :54 */
+    if (n <= 0) return ORACLE_ERR_INVALID_VALUE;
+    if (!N || !x || !rhs) return ORACLE_ERR_INVALID_POINTER;
+    if (kernel < 0 || kernel > 5) return ORACLE_ERR_INVALID_VALUE;
+    if (threads < 1) threads = 1;
+    if (threads > n) threads = n;
+
+    /* Row ranges of about equal pair counts, one per thread; each thread
+     * accumulates into its own double-double bins and |contribution| sums,
+     * merged afterwards in thread order.  Double-double sums are exact to
+     * ~2^-106 relative, so the result does not depend on the thread count
+     * beyond that. */
+    dd_t* acc = (dd_t*)calloc((size_t)n * (size_t)threads, sizeof(dd_t));
+    double* abs_t = abs_sum ? (double*)calloc((size_t)n * (size_t)threads, sizeof(double))
+                            : NULL;
+    dd_t* defect_t = (dd_t*)calloc((size_t)threads, sizeof(dd_t));
+    int* bound = (int*)malloc(sizeof(int) * (size_t)(threads + 1));
+    if (!acc || (abs_sum && !abs_t) || !defect_t || !bound) {
+        free(acc); free(abs_t); free(defect_t); free(bound);
+        return ORACLE_ERR_NO_MEMORY;
+    }
+    /* rows j >= b hold (n - b)(n - b + 1)/2 pairs */
+    const double total = 0.5 * (double)n * (double)(n + 1);
+    bound[0] = 0;
+    for (int t = 1; t < threads; ++t) {
+        const double rest = total * (double)(threads - t) / (double)threads;
+        int b = n - (int)floor(sqrt(2.0 * rest));
+        if (b < bound[t - 1]) b = bound[t - 1];
+        if (b > n) b = n;
+        bound[t] = b;
+    }
+    bound[threads] = n;
+#pragma omp parallel for num_threads(threads) schedule(static, 1)
+    for (int t = 0; t < threads; ++t) {
+        sink_t s;
+        s.acc = acc + (size_t)t * (size_t)n;
+        s.abs = abs_t ? abs_t + (size_t)t * (size_t)n : NULL;
+        agg_rows(n, N, x, kernel, beta0, beta_br, beta_sh, log_x0, inv_log_r, bracket_mode,
+                 bound[t], bound[t + 1], &s, &defect_t[t]);
+    }
+    dd_t defect = {0.0, 0.0};
+    for (int i = 0; i < n; ++i) {
+        dd_t a = {0.0, 0.0};
+        double ab = 0.0;
+        for (int t = 0; t < threads; ++t) {
+            const dd_t* p = &acc[(size_t)t * (size_t)n + i];
+            dd_add(&a, p->hi);
+            dd_add(&a, p->lo);
+            if (abs_t) ab += abs_t[(size_t)t * (size_t)n + i];
+        }
+        rhs[i] += dd_value(&a);
+        if (abs_sum) abs_sum[i] += ab;
+    }
+    for (int t = 0; t < threads; ++t) {
+        dd_add(&defect, defect_t[t].hi);
+        dd_add(&defect, defect_t[t].lo);
+    }
     if (m1_defect) *m1_defect = dd_value(&defect);
-    free(s.acc);
+    free(acc); free(abs_t); free(defect_t); free(bound);
     return 0;
+}
+
+int oracle_aggregation_rhs(int n, const double* N, const double* x,
+                           int kernel, double beta0, double beta_br,
+                           double beta_sh, double log_x0, double inv_log_r,
+                           int bracket_mode, double* rhs, double* abs_sum,
+                           double* m1_defect)
+{
+    return oracle_aggregation_rhs_threads(n, N, x, kernel, beta0, beta_br, beta_sh, log_x0,
+                                          inv_log_r, bracket_mode, rhs, abs_sum, m1_defect, 1);
 }
 
 /* ------------------------------------------------------------------------ */

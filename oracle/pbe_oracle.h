@@ -31,6 +31,14 @@ int oracle_aggregation_rhs(int n, const double* N, const double* x,
                            int bracket_mode, double* rhs, double* abs_sum,
                            double* m1_defect);
 
+/* The same with the rows split over `threads` OpenMP threads (equal pair
+ * counts), the per-thread double-double bins merged in thread order. */
+int oracle_aggregation_rhs_threads(int n, const double* N, const double* x,
+                                   int kernel, double beta0, double beta_br,
+                                   double beta_sh, double log_x0, double inv_log_r,
+                                   int bracket_mode, double* rhs, double* abs_sum,
+                                   double* m1_defect, int threads);
+
 int oracle_breakage_rhs(int n, int n_quad, const double* N, const double* x,
                         const double* t_q, const double* bw_q, int selection,
                         double S0, double v_ref, double alpha, double v_min,

@@ -35,9 +35,25 @@ def _stream():
     return torch.cuda.current_stream().cuda_stream
 
 
+def _check_inputs(N0, theta, prob):
+    """Refuse anything the device kernels would read out of bounds or from
+    host memory: N0 must be a float64 tensor on the HIP device with one entry
+    per pivot, theta must hold the 4 fitted parameters."""
+    if not isinstance(N0, torch.Tensor) or not N0.is_cuda:
+        raise ValueError("PBESolve: N0 must be a tensor on the HIP device")
+    if N0.dtype != torch.float64:
+        raise ValueError(f"PBESolve: N0 must be float64, got {N0.dtype}")
+    if N0.numel() != len(prob.x):
+        raise ValueError(f"PBESolve: N0 has {N0.numel()} entries, the grid {len(prob.x)}")
+    if not isinstance(theta, torch.Tensor) or theta.numel() != len(_SLOTS):
+        raise ValueError("PBESolve: theta must be a tensor of 4 parameters "
+                         "(beta_br, beta_sh, S0, alpha)")
+
+
 class PBESolve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, N0, theta, prob):
+        _check_inputs(N0, theta, prob)
         th = theta.detach().to("cpu", torch.float64).numpy()
         agg, brk = problems.with_params(prob.agg, prob.brk, th)
         adj = pbe.Adjoint(prob.x, prob.dt, prob.steps, prob.every, agg=agg, brk=brk,
@@ -50,6 +66,8 @@ class PBESolve(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, gN):
+        if not gN.is_cuda or gN.numel() != ctx.adj.n:
+            raise ValueError("PBESolve: the output gradient must be a device tensor of n entries")
         gN = gN.detach().to(torch.float64).contiguous()
         gN0 = torch.empty_like(gN)
         gth = torch.empty(pbe_grad_count(), dtype=torch.float64, device=gN.device)

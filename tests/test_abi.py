@@ -168,3 +168,17 @@ def test_aggregation_option_errors(pbe):
         opt = pbe.AggOptions(u, w)
         assert lib.pbe_launch_aggregation_rhs_ex(FAKE, FAKE, FAKE, ctypes.byref(A),
                                                  ctypes.byref(opt), None) == INVALID_VALUE
+
+
+def test_torch_solve_rejects_host_tensors():
+    """PBESolve refuses a host tensor before any device work (its pointer
+    would otherwise reach the kernels)."""
+    torch = pytest.importorskip("torch")
+    from pbe_hip import problems as P
+    from pbe_hip.torch_ops import PBESolve, Problem
+    x, N0, agg, brk, t_q, bw_q = P.config_adjoint(64)
+    prob = Problem(x, agg, brk, t_q, bw_q, 0.5, 2)
+    theta = torch.tensor([agg["beta_br"], agg["beta_sh"], brk["S0"], brk["alpha"]],
+                         dtype=torch.float64)
+    with pytest.raises(ValueError, match="HIP device"):
+        PBESolve.apply(torch.tensor(N0), theta, prob)

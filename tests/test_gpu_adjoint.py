@@ -133,6 +133,34 @@ def test_adjoint_config5_vs_central_fd(pbe):
         assert abs(gth[slot] - fd) <= 1e-5 * abs(fd) + 1e-9 * abs(L) / abs(th0[p]), p
 
 
+@pytest.mark.parametrize("every", [50, 0])
+def test_adjoint_config5_vs_complex_step_fixture(pbe, every):
+    """Config 5 at full size (n = 1024, r = 1.03, 2000 RK4 steps of 0.5 s):
+    the GPU adjoint, with checkpoints every 50 steps and with every stage
+    kept, against the complex-step gradient of the same discrete map computed
+    by the CPU oracle (tests/golden/adjoint_config5_cs.json, written by
+    tests/golden/make_adjoint_golden.py): the four parameter derivatives and
+    one direction of dL/dN0, each to 1e-8 relative."""
+    import json
+    import os
+    from pbe_hip import problems as P
+    with open(os.path.join(os.path.dirname(__file__), "golden", "adjoint_config5_cs.json")) as f:
+        fx = json.load(f)
+    x, N0, agg, brk, t_q, bw_q = P.config_adjoint(fx["n"])
+    assert np.allclose([agg["beta_br"], agg["beta_sh"], brk["S0"], brk["alpha"]],
+                       fx["theta0"], rtol=0, atol=0)
+    logT, mask = np.array(fx["logT"]), np.array(fx["mask"])
+    L, gN0, gth = _adjoint_grad(pbe, x, N0, agg, brk, t_q, bw_q, fx["dt"], fx["steps"], every,
+                                logT, mask, fx["eps"])
+    assert abs(L - fx["L"]) <= 1e-10 * abs(fx["L"]), (L, fx["L"])
+    for p, slot in enumerate((1, 2, 3, 4)):
+        ref = fx["grad"][P.FIT_PARAMS[p]]
+        print(P.FIT_PARAMS[p], gth[slot], ref)
+        assert abs(gth[slot] - ref) <= 1e-8 * abs(ref), (P.FIT_PARAMS[p], gth[slot], ref)
+    g_dir = float(gN0 @ np.array(fx["dN0"]))
+    assert abs(g_dir - fx["grad_dN0"]) <= 1e-8 * abs(fx["grad_dN0"]), (g_dir, fx["grad_dN0"])
+
+
 def _batch_small(pbe, count, seed, n_max):
     from pbe_hip import problems as P
     inst, N, x, t_q, bw_q = P.config_batched_fit(count, seed=seed, n_min=64, n_max=n_max)

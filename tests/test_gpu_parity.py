@@ -7,6 +7,8 @@ contributions to bin i; bins with A_i = 0 must be exactly 0) and the first
 moment sum_i x_i rhs_i within 1e-13 * sum_i x_i A_i of the oracle's exact
 clip/daughter defect.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -14,17 +16,20 @@ from parity import assert_m1, assert_parity
 
 pytestmark = pytest.mark.gpu
 
+# host threads for the exact oracle at large n (the GPU box's CPU share is 16)
+ORACLE_THREADS = max(1, min(16, os.cpu_count() or 1))
+
 
 def _gpu(pbe, N, x, agg=None, brk=None, t_q=None, bw_q=None):
     return pbe.rhs_on_device(N, x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
 
 
-def _oracle(orc, N, x, agg=None, brk=None, t_q=None, bw_q=None):
+def _oracle(orc, N, x, agg=None, brk=None, t_q=None, bw_q=None, threads=1):
     ref = np.zeros(len(N))
     A = np.zeros(len(N))
     D = 0.0
     if agg is not None:
-        r, a, d = orc.aggregation_rhs(N, x, **agg)
+        r, a, d = orc.aggregation_rhs(N, x, threads=threads, **agg)
         ref += r
         A += a
         D += d
@@ -172,46 +177,39 @@ def test_accumulates_into_rhs(pbe, oracle_mod):
     assert np.all(np.abs(got - base - ref) <= 1e-12 * A + slack)
 
 
-def test_flocculation_16384_vs_port(pbe, oracle_mod):
-    """Full-size workload: every entry against the OpenMP port (plain FP64,
-    reference bracket rule) and the first-moment identity."""
+@pytest.mark.parametrize("daughter", ["symmetric", "uniform"])
+def test_flocculation_16384(pbe, oracle_mod, daughter):
+    """Full-size workload (n = 16384, r = 1.02, Brownian+shear, threshold
+    breakage) with symmetric-binary daughters and with the 16-point
+    Gauss-Legendre uniform table: every entry against the exact oracle at
+    1e-12 A_i and the first-moment identity at 1e-13, and against the OpenMP
+    port (plain FP64, the reference's bracket rule)."""
     from pbe_hip import problems as P
-    x, N, agg, brk, t_q, bw_q = P.config_flocculation(16384)
+    x, N, agg, brk, t_q, bw_q = P.config_flocculation(16384, daughter=daughter, n_quad=16)
     got = _gpu(pbe, N, x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
-    port = oracle_mod.port_aggregation_rhs(N, x, threads=16, **agg)
-    port += oracle_mod.port_breakage_rhs(N, x, t_q, bw_q, threads=16, **brk)
+    port = oracle_mod.port_aggregation_rhs(N, x, threads=ORACLE_THREADS, **agg)
+    port += oracle_mod.port_breakage_rhs(N, x, t_q, bw_q, threads=ORACLE_THREADS, **brk)
     scale = np.abs(port).max()
     assert np.all(np.abs(got - port) <= 1e-10 * scale + 1e-9 * np.abs(port))
-    # the exact oracle on the same input (double-double, ~10 s on one core)
-    ref, A, D = _oracle(oracle_mod, N, x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
-    w = assert_parity(got, ref, A, what="n=16384")
-    m = assert_m1(got, x, A, D, what="n=16384")
-    print(f"n=16384: entry {w:.2e} m1 {m:.2e}")
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q,
+                        threads=ORACLE_THREADS)
+    w = assert_parity(got, ref, A, what=f"n=16384 {daughter}")
+    m = assert_m1(got, x, A, D, what=f"n=16384 {daughter}")
+    print(f"n=16384 {daughter} (n_quad {len(t_q)}): entry {w:.2e} m1 {m:.2e}")
 
 
-def _vs_port(got, port, x, what):
-    """Every entry against the OpenMP port, as for n = 16384, and the first
-    moment: sum x got within 1e-13 of sum x |port| (sum x A is 3.2 times
-    larger at n = 32768, so this is the stricter bound)."""
-    scale = np.abs(port).max()
-    err = np.abs(got - port) / (1e-10 * scale + 1e-9 * np.abs(port))
-    assert err.max() <= 1.0, f"{what}: bin {int(np.argmax(err))} off the port"
-    m1 = abs(float(np.sum(x * got))) / float(np.sum(x * np.abs(port)))
-    assert m1 <= 1e-13, f"{what}: |sum x rhs| / sum x |port| = {m1:.3e}"
-    return m1
-
-
-@pytest.mark.parametrize("n,r,search,exact", [(32768, 1.01, False, True),
-                                              (40007, 1.008, True, False),
-                                              (65536, 1.005, False, False)])
-def test_beyond_int32_pair_count(pbe, oracle_mod, n, r, search, exact):
+@pytest.mark.parametrize("n,r,search", [(32768, 1.01, False), (40007, 1.008, True),
+                                        (65536, 1.005, False)])
+def test_beyond_int32_pair_count(pbe, oracle_mod, n, r, search):
     """Sizes past the reference's n = 23,169 (its triangle decode computes
     (2n+1)^2 in int, src/aggregation_kernels.cuh:171): 5.4e8 to 2.1e9 pairs
     through the band sweep and the column strips (64-bit plan).  Geometric
     grids with the log-ratio lookup, and a ragged n (157 column blocks, the
-    last partial) with the binary search; aggregation and breakage checked
-    apart (their magnitudes differ by 35 decades here), at n = 32768 also
-    against the exact oracle."""
+    last partial) with the binary search.  Aggregation and breakage are
+    checked apart (their magnitudes differ by 35 decades here), each entry
+    against the exact oracle at 1e-12 A_i and the first moment at 1e-13.
+    n = 65536 at r = 1.005 has a near-pair reach of 1062 bins, so the strips
+    hold per-pair near chunks with brackets beyond their LDS windows."""
     from pbe_hip import problems as P
     x = P.geometric_grid(n, 1e-18, r)
     N = P.lognormal_ic(x, x[n // 4], 0.8, 1e12, noise=0.05, seed=n)
@@ -222,16 +220,40 @@ def test_beyond_int32_pair_count(pbe, oracle_mod, n, r, search, exact):
     t_q, bw_q = P.daughter_symmetric()
     ga = _gpu(pbe, N, x, agg=agg)
     gb = _gpu(pbe, N, x, brk=brk, t_q=t_q, bw_q=bw_q)
-    ma = _vs_port(ga, oracle_mod.port_aggregation_rhs(N, x, threads=16, **agg), x,
-                  f"aggregation n={n}")
-    mb = _vs_port(gb, oracle_mod.port_breakage_rhs(N, x, t_q, bw_q, threads=16, **brk), x,
-                  f"breakage n={n}")
-    print(f"n={n}: m1 aggregation {ma:.2e} breakage {mb:.2e}")
-    if exact:
-        ref, A, D = _oracle(oracle_mod, N, x, agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
-        w = assert_parity(ga + gb, ref, A, what=f"n={n}")
-        m = assert_m1(ga + gb, x, A, D, what=f"n={n}")
-        print(f"n={n} exact oracle: entry {w:.2e} m1 {m:.2e}")
+    ra, Aa, Da = _oracle(oracle_mod, N, x, agg=agg, threads=ORACLE_THREADS)
+    rb, Ab, Db = _oracle(oracle_mod, N, x, brk=brk, t_q=t_q, bw_q=bw_q)
+    wa = assert_parity(ga, ra, Aa, what=f"aggregation n={n}")
+    ma = assert_m1(ga, x, Aa, Da, what=f"aggregation n={n}")
+    wb = assert_parity(gb, rb, Ab, what=f"breakage n={n}")
+    mb = assert_m1(gb, x, Ab, Db, what=f"breakage n={n}")
+    print(f"n={n}: aggregation entry {wa:.2e} m1 {ma:.2e}; breakage entry {wb:.2e} m1 {mb:.2e}")
+
+
+@pytest.mark.parametrize("n,r,kernel", [(9000, 1.002, 5), (9000, 1.005, 5), (9000, 1.005, 0),
+                                         (20000, 1.004, 3)])
+def test_wide_strips_fine_grid(pbe, oracle_mod, n, r, kernel):
+    """512-column blocks (tile_u = 8) on geometric grids whose near-pair reach
+    (1062 bins at r = 1.005, 2656 at 1.002) exceeds the block and its LDS
+    window: strip chunks take the per-pair near path with brackets found
+    beyond the 768-bin window (log-ratio guess plus fix-up).  Every entry
+    against the exact oracle."""
+    from pbe_hip import hip, problems as P
+    x = P.geometric_grid(n, 1e-18, r)
+    N = P.lognormal_ic(x, x[n // 3], 1.5, 1e12, noise=0.05, seed=n + kernel)
+    log_x0, inv_log_r = P.grid_lookup(x)
+    agg = dict(kernel=kernel, beta0=1e-17, beta_br=6.73e-18, beta_sh=1e-3, log_x0=log_x0,
+               inv_log_r=inv_log_r)
+    dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
+    drhs = hip.DeviceArray(n)
+    drhs.zero()
+    hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, pbe.agg_params(n, agg), tile_u=8,
+                                            band_split=0))
+    hip.synchronize()
+    got = drhs.download()
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg, threads=ORACLE_THREADS)
+    what = f"U=8 n={n} r={r} kernel {kernel}"
+    assert_parity(got, ref, A, what=what)
+    assert_m1(got, x, A, D, what=what)
 
 
 @pytest.mark.parametrize("u", [1, 2, 4, 8])
