@@ -154,6 +154,22 @@ hipError_t pbe_launch_breakage_rhs_deterministic(const double* N, const double* 
                                                  void* workspace, size_t workspace_bytes,
                                                  hipStream_t stream);
 
+/* pbe_launch_aggregation_rhs_deterministic without a workspace argument and
+ * without any allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under hipGraph stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The workspace is a
+ * static device array inside the library sized for n up to
+ * PBE_DETERMINISTIC_NOALLOC_MAX_N (about 23 MB per device, held until the
+ * process ends); n above that returns hipErrorInvalidValue before any launch.
+ * All calls on one device share that array: calls that overlap on one device
+ * (different streams or threads) race and must be serialised by the caller.
+ * Same results, bit for bit, as the workspace form; other errors as there. */
+#define PBE_DETERMINISTIC_NOALLOC_MAX_N 65536
+hipError_t pbe_launch_aggregation_rhs_deterministic_noalloc(const double* N, const double* x,
+                                                            double* rhs,
+                                                            const pbe_aggregation_params* params,
+                                                            hipStream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Device time integration (no counterpart in the reference library: it     */
 /* replaces the host-side euler_step / rk4_step of the reference's example  */

@@ -162,6 +162,20 @@ extern "C" hipError_t pbe_launch_aggregation_rhs_deterministic(
     return hipGetLastError();
 }
 
+extern "C" hipError_t pbe_launch_aggregation_rhs_deterministic_noalloc(
+    const double* N, const double* x, double* rhs, const pbe_aggregation_params* p,
+    hipStream_t stream)
+{
+    if (!p || p->n <= 0) return hipErrorInvalidValue;
+    if (!N || !x || !rhs) return hipErrorInvalidDevicePointer;
+    if (p->kernel_type < 0 || p->kernel_type > 5) return hipErrorInvalidValue;
+    if (p->block_size > 1024) return hipErrorInvalidConfiguration;
+    if (p->n > PBE_DETERMINISTIC_NOALLOC_MAX_N) return hipErrorInvalidValue;
+    hipError_t e = pbe::launch_aggregation_deterministic_noalloc(N, x, rhs, *p, stream);
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
 extern "C" hipError_t pbe_launch_breakage_rhs_deterministic(
     const double* N, const double* x, const double* t_q, const double* bw_q, double* rhs,
     const pbe_breakage_params* p, void* workspace, size_t workspace_bytes, hipStream_t stream)

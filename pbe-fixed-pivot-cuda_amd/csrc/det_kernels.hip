@@ -26,6 +26,7 @@
 // fragment volume); the fragments clipped to bin 0 or bin n-1 are summed by
 // the wave that owns that bin.
 #include <algorithm>
+#include <mutex>
 
 #include "pbe_device.hpp"
 #include "pbe_internal.hpp"
@@ -49,10 +50,17 @@ struct DetLayout {
 };
 
 // Row segments of the far partials: a function of n only.
-inline int det_segments(int n)
+constexpr int det_segments(int n)
 {
-    const int s = (n + 1023) / 1024;
-    return std::min(16, std::max(1, s));
+    return (n + 1023) / 1024 < 1 ? 1 : ((n + 1023) / 1024 > 16 ? 16 : (n + 1023) / 1024);
+}
+
+// Workspace of the aggregation gather: the table (the stride of the widest
+// kernel), the per-block column sums and the lo / hi far partials.
+constexpr size_t det_agg_doubles(int n)
+{
+    return (size_t)n * DetLayout<5>::TS + (size_t)((n + kDetBlock - 1) / kDetBlock) * Kern<5>::R +
+           2 * (size_t)det_segments(n) * (size_t)n;
 }
 
 struct DetAggArgs {
@@ -521,11 +529,7 @@ void det_brk_launch(const DetBrkArgs& a, hipStream_t s)
 
 }  // namespace
 
-size_t det_aggregation_workspace_doubles(int n)
-{
-    const size_t nbq = (size_t)(n + kDetBlock - 1) / kDetBlock;
-    return (size_t)n * DetLayout<5>::TS + nbq * Kern<5>::R + 2 * (size_t)det_segments(n) * n;
-}
+size_t det_aggregation_workspace_doubles(int n) { return det_agg_doubles(n); }
 
 size_t det_breakage_workspace_doubles(int n) { return (size_t)n; }
 
@@ -555,6 +559,57 @@ hipError_t launch_aggregation_deterministic(const double* N, const double* x, do
     case 5: return det_agg_launch<5>(a, stream);
     default: return hipErrorInvalidValue;
     }
+}
+
+// ---------------------------------------------------------------------------
+// The aggregation gather without a caller workspace, for callers that cannot
+// allocate device memory (code under graph capture, serving loops).  Its
+// workspace is `scratch`, a file-scope device array sized for n up to
+// kDetNoAllocMaxN (23 MB; each device holds its own copy until the process
+// ends).  Host code never uses the array's name as an address: the first call
+// on a device looks the device's copy up with hipGetSymbolAddress and keeps
+// the pointer per device.  Every call on a device shares its copy, so calls
+// that overlap on one device (other streams, other threads) race.  The
+// kernels overwrite every entry they read, so nothing needs clearing between
+// calls.
+// ---------------------------------------------------------------------------
+constexpr size_t kDetNoAllocDoubles = det_agg_doubles(kDetNoAllocMaxN);
+static __device__ __align__(256) double scratch[kDetNoAllocDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+hipError_t scratch_on_current_device(double** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+}  // namespace
+
+hipError_t launch_aggregation_deterministic_noalloc(const double* N, const double* x,
+                                                    double* rhs,
+                                                    const pbe_aggregation_params& p,
+                                                    hipStream_t stream)
+{
+    if (p.n > kDetNoAllocMaxN) return hipErrorInvalidValue;
+    double* ws = nullptr;
+    hipError_t e = scratch_on_current_device(&ws);
+    if (e != hipSuccess) return e;
+    return launch_aggregation_deterministic(N, x, rhs, p, ws, stream);
 }
 
 hipError_t launch_breakage_deterministic(const double* N, const double* x, const double* t_q,

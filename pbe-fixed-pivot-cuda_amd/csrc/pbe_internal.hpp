@@ -29,6 +29,13 @@ size_t det_breakage_workspace_doubles(int n);
 hipError_t launch_aggregation_deterministic(const double* N, const double* x, double* rhs,
                                             const pbe_aggregation_params& p, double* ws,
                                             hipStream_t stream);
+// The aggregation gather on the library's static per-device workspace (no
+// allocation); n <= kDetNoAllocMaxN.
+constexpr int kDetNoAllocMaxN = PBE_DETERMINISTIC_NOALLOC_MAX_N;
+hipError_t launch_aggregation_deterministic_noalloc(const double* N, const double* x,
+                                                    double* rhs,
+                                                    const pbe_aggregation_params& p,
+                                                    hipStream_t stream);
 hipError_t launch_breakage_deterministic(const double* N, const double* x, const double* t_q,
                                          const double* bw_q, double* rhs,
                                          const pbe_breakage_params& p, double* ws,

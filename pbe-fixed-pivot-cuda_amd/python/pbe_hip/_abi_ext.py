@@ -13,6 +13,7 @@ def declare(lib):
         "pbe_breakage_deterministic_workspace_bytes": (sz, [i]),
         "pbe_launch_aggregation_rhs_deterministic": (i, [vp, vp, vp, vp, vp, sz, vp]),
         "pbe_launch_breakage_rhs_deterministic": (i, [vp, vp, vp, vp, vp, vp, vp, sz, vp]),
+        "pbe_launch_aggregation_rhs_deterministic_noalloc": (i, [vp, vp, vp, vp, vp]),
         "pbe_integrator_workspace_bytes": (sz, [i, i]),
         "pbe_integrate": (i, [vp, vp, vp, vp, vp, vp, d, i, i, vp, sz, vp]),
         "pbe_integrator_graph_create": (i, [ctypes.POINTER(vp), vp, vp, vp, vp, vp, vp, d,

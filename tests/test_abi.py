@@ -182,3 +182,18 @@ def test_torch_solve_rejects_host_tensors():
                          dtype=torch.float64)
     with pytest.raises(ValueError, match="HIP device"):
         PBESolve.apply(torch.tensor(N0), theta, prob)
+
+
+def test_noalloc_argument_errors(pbe):
+    """The no-allocation gather checks its arguments like the workspace form,
+    and refuses n beyond its static array, before any HIP call."""
+    lib = pbe.library()
+    f = lib.pbe_launch_aggregation_rhs_deterministic_noalloc
+    assert f(FAKE, FAKE, FAKE, None, None) == INVALID_VALUE
+    assert f(FAKE, FAKE, FAKE, ctypes.byref(pbe.AggregationParams(n=0)), None) == INVALID_VALUE
+    assert f(None, FAKE, FAKE, ctypes.byref(pbe.AggregationParams(n=10)), None) == \
+        INVALID_DEVICE_POINTER
+    assert f(FAKE, FAKE, FAKE, ctypes.byref(pbe.AggregationParams(n=10, kernel_type=7)),
+             None) == INVALID_VALUE
+    assert f(FAKE, FAKE, FAKE, ctypes.byref(pbe.AggregationParams(n=65537)), None) == \
+        INVALID_VALUE
