@@ -173,8 +173,9 @@ void agg_tiles_kernel(AggArgs a, int S)
     }
 }
 
-// Rows below the band, as column strips over the resident grid: one
-// workgroup of StripWaves<U> waves per compute unit.
+// Column strips over the resident grid (with or without each block's
+// diagonal band): one workgroup of StripWaves<U> waves per compute unit,
+// each taking an equal share of the plan's cost line.
 template <int KT, int U>
 __global__ __launch_bounds__(kWave * StripWaves<U>::value, 1)
 void agg_strip_kernel(AggArgs a, StripPlan plan)
@@ -182,8 +183,8 @@ void agg_strip_kernel(AggArgs a, StripPlan plan)
     __shared__ StripSmem<KT, U> sm;
     const long long CT = strip_cost_total(plan), G = gridDim.x, g = blockIdx.x;
     int qa, ca, qb, cb;
-    strip_first_unit(plan, strip_chunk_at_cost(plan, CT * g / G), qa, ca);
-    strip_first_unit(plan, strip_chunk_at_cost(plan, CT * (g + 1) / G), qb, cb);
+    strip_chunk_at_cost(plan, CT * g / G, qa, ca);
+    strip_chunk_at_cost(plan, CT * (g + 1) / G, qb, cb);
     // visits: blocks qa .. qe - 1 (the last one ends at chunk cb of block qb)
     if (qa > qb || (qa == qb && ca >= cb)) return;   // empty share
     const int qe = cb > 0 ? qb + 1 : qb;
@@ -192,7 +193,7 @@ void agg_strip_kernel(AggArgs a, StripPlan plan)
     for (int i = threadIdx.x; i < StripSmem<KT, U>::WIN; i += blockDim.x) sm.colwin[i] = 0.0;
     for (int q = qa, c = ca; q < qe; ++q, c = 0) {
         const int hi = (q == qb) ? cb : strip_nc(plan, q);
-        agg_strip_visit<KT, U>(a, q, c, hi, q + 1 < qe ? q + 1 : -1, cols, sm);
+        agg_strip_visit<KT, U>(a, plan, q, c, hi, q + 1 < qe ? q + 1 : -1, cols, sm);
     }
 }
 
@@ -222,8 +223,8 @@ void agg_fused_kernel(AggArgs a, int nseg, BandSegs segs, StripPlan plan, int ns
     }
     const long long CT = strip_cost_total(plan), G = gridDim.x - nsweep, g = blockIdx.x - nsweep;
     int qa, ca, qb, cb;
-    strip_first_unit(plan, strip_chunk_at_cost(plan, CT * g / G), qa, ca);
-    strip_first_unit(plan, strip_chunk_at_cost(plan, CT * (g + 1) / G), qb, cb);
+    strip_chunk_at_cost(plan, CT * g / G, qa, ca);
+    strip_chunk_at_cost(plan, CT * (g + 1) / G, qb, cb);
     if (qa > qb || (qa == qb && ca >= cb)) return;   // empty share
     const int qe = cb > 0 ? qb + 1 : qb;
     StripCols<KT, U, kFusedWaves> cols;
@@ -232,7 +233,7 @@ void agg_fused_kernel(AggArgs a, int nseg, BandSegs segs, StripPlan plan, int ns
         sm.strip.colwin[i] = 0.0;
     for (int q = qa, c = ca; q < qe; ++q, c = 0) {
         const int hi = (q == qb) ? cb : strip_nc(plan, q);
-        agg_strip_visit<KT, U, kFusedWaves>(a, q, c, hi, q + 1 < qe ? q + 1 : -1, cols,
+        agg_strip_visit<KT, U, kFusedWaves>(a, plan, q, c, hi, q + 1 < qe ? q + 1 : -1, cols,
                                              sm.strip);
     }
 }
@@ -282,17 +283,44 @@ static int resident_groups()
     return v;
 }
 
+// Strip plan of an n-bin problem in blocks of T = 64 U columns (the rows
+// below each block's diagonal band; `band`: the band's rows too).  Chunk
+// costs on the workgroups' cost line: a far chunk 4, a near chunk 40 (a near
+// row is a chain of dependent steps with a bracket move in most of them: on
+// the MI355X it takes about ten times a far row), a diagonal chunk 6, a
+// block's visit 20.  On a geometric grid the strip chunks within the
+// near-pair reach log_r(1 / (r - 1)) bins below a block's band are the near
+// ones; measured on the strip launch at r = 1.005: n = 16384 231 us with the
+// near chunks charged 10, 107 us at 40; n = 65536 (512-column blocks)
+// 1685 us at 10, 674 us at 40.
+static StripPlan make_strip_plan(int n, int U, bool band, double inv_log_r)
+{
+    StripPlan plan;
+    const int T = kWave * U;
+    plan.U = U;
+    plan.nb = (n + T - 1) / T;
+    plan.band = band ? 1 : 0;
+    plan.m_near = 0;
+    if (inv_log_r > 0.0 && std::isfinite(inv_log_r)) {
+        const double reach = -std::log(std::expm1(1.0 / inv_log_r)) * inv_log_r;
+        if (reach > T && reach < 1e7) plan.m_near = (int)std::ceil((reach - T) / kStripRows);
+    }
+    plan.cf = 4;
+    plan.cn = 40;
+    plan.cd = 6;
+    plan.cv = 20;
+    plan.total = strip_total(plan);
+    return plan;
+}
+
 template <int KT, int U>
 static hipError_t launch_agg_t(AggArgs args, int split, hipStream_t stream)
 {
     constexpr int T = AggShape<U>::T;
     args.nb = (args.n + T - 1) / T;
     const long long band_items = (long long)split * band_tiles(args.nb);
-    StripPlan plan;
-    plan.U = U;
-    plan.nb = args.nb;
-    plan.total = strip_total(plan);
-    if (T == kBandT && args.nb <= kFusedMaxBlocks && plan.total > 0) {
+    const StripPlan sep = make_strip_plan(args.n, U, false, args.inv_log_r);
+    if (T == kBandT && args.nb <= kFusedMaxBlocks && sep.total > 0) {
         // mid sizes: sweep items on at most half of the resident slots of
         // the fused kernel, strip shares on the rest (empty shares return)
         const long long Gf = fused_resident_groups<KT>();
@@ -302,12 +330,12 @@ static hipError_t launch_agg_t(AggArgs args, int split, hipStream_t stream)
         if (nsweep * 2 <= Gf) {
             hipLaunchKernelGGL((agg_fused_kernel<KT>), dim3((unsigned)Gf),
                                dim3(kWave * kFusedWaves), 0, stream, args, Sf,
-                               band_segments_cached(Sf, args.inv_log_r), plan, (int)nsweep);
+                               band_segments_cached(Sf, args.inv_log_r), sep, (int)nsweep);
             return hipSuccess;
         }
     }
     const long long G = resident_groups<KT, U>();
-    if (plan.total < G) {
+    if (sep.total < G) {
         // too little far work for the strips: one launch of all tiles
         const long long tiles = (long long)args.nb * (args.nb + 1) / 2;
         const long long items = band_items + tiles - band_tiles(args.nb);
@@ -326,7 +354,7 @@ static hipError_t launch_agg_t(AggArgs args, int split, hipStream_t stream)
                            stream, args, split);
     }
     hipLaunchKernelGGL((agg_strip_kernel<KT, U>), dim3((unsigned)G),
-                       dim3(kWave * StripWaves<U>::value), 0, stream, args, plan);
+                       dim3(kWave * StripWaves<U>::value), 0, stream, args, sep);
     return hipSuccess;
 }
 

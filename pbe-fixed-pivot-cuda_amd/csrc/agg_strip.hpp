@@ -38,24 +38,71 @@ struct StripWaves {
 };
 constexpr int kStripExtra = 256;     // near-birth reach of the LDS window beyond the block
 
-// Chunk layout of one launch: block q >= 2 has (q - 1) U chunks.
+// Chunk layout of one launch.  Chunk c of column block q holds the rows
+// [64 c, 64 c + 64) (clipped to the rows j <= the block's last column).
+// Without the band (`band` = 0: a separate launch takes each block's diagonal
+// band), block q >= 2 has (q - 1) U chunks: its rows below the band.  With the
+// band, block q has (q + 1) U chunks: the rows below the band, then the band's
+// lower tile (rows [(q-1)T, qT)), then the diagonal tile (rows [qT, (q+1)T)).
+// Chunks [sf, se) are near chunks (the band's lower tile, and on fine grids
+// the m_near strip chunks just below it), chunks [se, nc) diagonal chunks;
+// the rest are far chunks.  The shares of the launch split a cost line on
+// which a far chunk costs cf, a near chunk cn, a diagonal chunk cd and each
+// block's visit (column set-up and window flush) cv.
 struct StripPlan {
-    long long total;   // C: number of chunks
+    long long total;   // chunks of all blocks
     int nb;            // column blocks
     int U;             // chunks per block height (T / 64)
+    int band;          // 1: blocks hold their diagonal band rows too
+    int m_near;        // strip chunks below the band charged as near
+    int cf, cn, cd, cv;
 };
 
 __host__ __device__ __forceinline__ int strip_nc(const StripPlan& s, int q)
 {
-    return q >= 2 ? (q - 1) * s.U : 0;
+    return s.band ? (q + 1) * s.U : (q >= 2 ? (q - 1) * s.U : 0);
 }
 
-// Chunks of the blocks before q.
+// first diagonal chunk of block q (nc without the band)
+__host__ __device__ __forceinline__ int strip_se(const StripPlan& s, int q)
+{
+    return s.band ? q * s.U : strip_nc(s, q);
+}
+
+// first near chunk of block q
+__host__ __device__ __forceinline__ int strip_sf(const StripPlan& s, int q)
+{
+    const int st = q >= 1 ? (q - 1) * s.U : 0;
+    return st > s.m_near ? st - s.m_near : 0;
+}
+
+__host__ __device__ __forceinline__ long long tri(long long m) { return m > 0 ? m * (m + 1) / 2 : 0; }
+
+// Sums over the blocks before q of nc, se and sf, and the number of visits.
 __host__ __device__ __forceinline__ long long strip_prefix(const StripPlan& s, int q)
 {
-    if (q <= 2) return 0;
-    const long long m = q - 2;
-    return (long long)s.U * m * (m + 1) / 2;
+    return s.band ? (long long)s.U * q * (q + 1) / 2 : s.U * tri(q - 2);
+}
+
+__host__ __device__ __forceinline__ long long strip_se_sum(const StripPlan& s, int q)
+{
+    return s.band ? (long long)s.U * q * (q - 1) / 2 : strip_prefix(s, q);
+}
+
+__host__ __device__ __forceinline__ long long strip_sf_sum(const StripPlan& s, int q)
+{
+    // sum over a = q' - 1 in [0, q - 2] of max(0, U a - m_near)
+    const long long A = q - 2, a0 = s.m_near / s.U + 1;
+    if (A < a0) return 0;
+    return (long long)s.U * (A * (A + 1) / 2 - a0 * (a0 - 1) / 2) -
+           (long long)s.m_near * (A - a0 + 1);
+}
+
+__host__ __device__ __forceinline__ long long strip_cost_start(const StripPlan& s, int q)
+{
+    const long long nc = strip_prefix(s, q), se = strip_se_sum(s, q), sf = strip_sf_sum(s, q);
+    const long long visits = s.band ? q : (q > 2 ? q - 2 : 0);
+    return s.cv * visits + s.cf * sf + s.cn * (se - sf) + s.cd * (nc - se);
 }
 
 __host__ __device__ __forceinline__ long long strip_total(const StripPlan& s)
@@ -63,56 +110,85 @@ __host__ __device__ __forceinline__ long long strip_total(const StripPlan& s)
     return strip_prefix(s, s.nb);
 }
 
-// Workgroup shares.  A visit's column set-up and window flush, during which
-// the workgroup's waves wait for one another, cost about five chunks of pairs,
-// so the shares split the cost line evenly: chunk t of block
-// q sits at cost t + kVisitCost (q - 1), block q's visit charge lying just
-// before its chunk 0.  A share that crosses into a new block carries that
-// block's charge and so holds one chunk fewer.
-constexpr int kVisitCost = 5;
-
-__host__ __device__ __forceinline__ long long strip_cost_start(const StripPlan& s, int q)
-{
-    return strip_prefix(s, q) + (long long)kVisitCost * (q - 2);
-}
-
 __host__ __device__ __forceinline__ long long strip_cost_total(const StripPlan& s)
 {
     return strip_cost_start(s, s.nb);
 }
 
-// First chunk at or after cost position f (s.total past the end).
-__device__ __forceinline__ long long strip_chunk_at_cost(const StripPlan& s, long long f)
+__host__ __device__ __forceinline__ long long ceil_div(long long a, long long b)
 {
-    if (f >= strip_cost_total(s)) return s.total;
-    // q = m + 2 with U m (m + 1) / 2 + kVisitCost m <= f
-    const double b = 0.5 * s.U + kVisitCost;
-    int m = (int)floor((sqrt(b * b + 2.0 * s.U * (double)f) - b) / s.U);
-    if (m < 0) m = 0;
-    while (m > 0 && strip_cost_start(s, m + 2) > f) --m;
-    while (m + 3 <= s.nb && strip_cost_start(s, m + 3) <= f) ++m;
-    const int q = m + 2;
-    long long off = f - strip_cost_start(s, q) - kVisitCost;
-    if (off < 0) off = 0;
-    return strip_prefix(s, q) + off;
+    return (a + b - 1) / b;
 }
 
-// Chunk (q, c) at linear index t; (nb, 0) past the end.
-__device__ __forceinline__ void strip_first_unit(const StripPlan& s, long long t, int& q, int& c)
+// First chunk (q, c) whose start on the cost line is at or after f;
+// (nb, 0) past the end.  Chunk c of block q starts at
+// strip_cost_start(q) + cv + the cost of chunks [0, c).
+__host__ __device__ __forceinline__ void strip_chunk_at_cost(const StripPlan& s, long long f,
+                                                            int& q, int& c)
 {
-    if (t >= s.total) {
+    if (f >= strip_cost_total(s)) {
         q = s.nb;
         c = 0;
         return;
     }
-    // q = m + 2 with U m (m + 1) / 2 <= t < U (m + 1)(m + 2) / 2
-    int m = (int)floor(0.5 * (sqrt(1.0 + 8.0 * (double)t / s.U) - 1.0));
-    if (m < 0) m = 0;
-    while (m > 0 && strip_prefix(s, m + 2) > t) --m;
-    while (strip_prefix(s, m + 3) <= t) ++m;
-    q = m + 2;
-    c = (int)(t - strip_prefix(s, q));
+    int lo = 0, hi = s.nb - 1;   // largest block with cost_start <= f
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (strip_cost_start(s, mid) <= f) lo = mid; else hi = mid - 1;
+    }
+    q = lo;
+    const int nc = strip_nc(s, q), se = strip_se(s, q), sf = strip_sf(s, q);
+    const long long off = f - strip_cost_start(s, q) - (nc > 0 ? s.cv : 0);
+    long long cc;
+    if (off <= 0) {
+        cc = 0;
+    } else if (off <= (long long)s.cf * sf) {
+        cc = ceil_div(off, s.cf);
+    } else if (off <= (long long)s.cf * sf + (long long)s.cn * (se - sf)) {
+        cc = sf + ceil_div(off - (long long)s.cf * sf, s.cn);
+    } else {
+        cc = se + ceil_div(off - (long long)s.cf * sf - (long long)s.cn * (se - sf), s.cd);
+    }
+    if (cc >= nc) {
+        q += 1;
+        cc = 0;
+    }
+    c = (int)cc;
 }
+
+// Rows of a visit of block q split by cost: row j weighs cf below row 64 sf,
+// cn below row 64 se and cd above.  row_at(t) is the first row whose cost
+// offset from r0 is at least t; cost(j) the offset of row j.
+struct VisitRows {
+    int r0, r1, b1, b2;
+    long long wf, wn, wd, c1, c2;
+    __host__ __device__ static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+    __host__ __device__ __forceinline__ VisitRows(const StripPlan& s, int q, int r0_, int r1_)
+    {
+        r0 = r0_;
+        r1 = r1_;
+        b1 = clampi(kStripRows * strip_sf(s, q), r0, r1);
+        b2 = clampi(kStripRows * strip_se(s, q), r0, r1);
+        wf = s.cf;
+        wn = s.cn;
+        wd = s.cd;
+        c1 = wf * (b1 - r0);
+        c2 = c1 + wn * (b2 - b1);
+    }
+    __host__ __device__ __forceinline__ long long total() const { return c2 + wd * (r1 - b2); }
+    __host__ __device__ __forceinline__ long long cost(int j) const
+    {
+        return j <= b1 ? wf * (j - r0) : (j <= b2 ? c1 + wn * (j - b1) : c2 + wd * (j - b2));
+    }
+    __host__ __device__ __forceinline__ int row_at(long long t) const
+    {
+        long long j;
+        if (t <= c1) j = r0 + ceil_div(t, wf);
+        else if (t <= c2) j = b1 + ceil_div(t - c1, wn);
+        else j = b2 + ceil_div(t - c2, wd);
+        return (int)(j < r0 ? r0 : (j > r1 ? r1 : j));
+    }
+};
 
 // Order this wave's LDS writes before its later LDS reads by other lanes.
 // LDS instructions of one wave execute in issue order, so only code motion by
@@ -133,11 +209,12 @@ struct StripSmem {
     double rowbuf[NWV][(kStripPieceRows + 2) * RS];   // per wave: row factors A_r(j) and x_j
     double colwin[WIN];                  // births/deaths of the window bins
     double xw[WIN + 1];                  // pivots of the window bins
-    double idw[T];                       // 1/(x_{i+1} - x_i) of the block's columns
+    double idw[WIN];                     // 1/(x_{i+1} - x_i) of the window bins
     double colc[T];                      // cube roots of the block's pivots
     double colic[T];                     // and their reciprocals
     double coln[T];                      // populations of the block's columns
-    int prog[NWV];                       // per wave: rows of the visit done so far
+    double sk[Kern<KT>::R][T + 1];       // suffix sums of B_r over the block's columns
+    int prog[NWV];                       // per wave: cost of the visit's rows done so far
 };
 
 // Column factors B_r(k) = N_k phi_r(k) of this lane's U columns.
@@ -174,7 +251,7 @@ struct StripCols {
     static constexpr int NTH = kWave * NWV;   // threads of the workgroup
     static constexpr int NW = (WIN + 1 + NTH - 1) / NTH;
     static constexpr int NT = (T + NTH - 1) / NTH;
-    double xv[NW], nv[NT], xn[NT];   // as loaded: apply valid_* before use
+    double xv[NW], xn[NW], nv[NT];   // as loaded: apply valid_* before use
 
     // Unconditional loads, indices clamped into the arrays: a load under a
     // branch makes the compiler wait for it (and for every vector-memory
@@ -183,13 +260,13 @@ struct StripCols {
     {
         const int tid = threadIdx.x, n = a.n, k0 = q * T;
 #pragma unroll
-        for (int t = 0; t < NW; ++t) xv[t] = a.x[min(k0 + tid + NTH * t, n - 1)];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int k = k0 + tid + NTH * t;
-            nv[t] = a.N[min(k, n - 1)];
-            xn[t] = a.x[min(k + 1, n - 1)];
+        for (int t = 0; t < NW; ++t) {
+            const int b = k0 + tid + NTH * t;
+            xv[t] = a.x[min(b, n - 1)];
+            xn[t] = a.x[min(b + 1, n - 1)];
         }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) nv[t] = a.N[min(k0 + tid + NTH * t, n - 1)];
     }
     // window pivot t of this thread (inf past the window or the grid)
     __device__ __forceinline__ double pivot(int n, int q, int t) const
@@ -202,20 +279,82 @@ struct StripCols {
         const int i = threadIdx.x + NTH * t, k = q * T + i;
         return (i < T && k < n) ? nv[t] : 0.0;
     }
+    // the pivot after window pivot t (inf past the window or the grid)
     __device__ __forceinline__ double next_pivot(int n, int q, int t) const
     {
-        const int i = threadIdx.x + NTH * t, k = q * T + i;
-        return (i < T && k + 1 < n) ? xn[t] : __builtin_inf();
+        const int i = threadIdx.x + NTH * t, b = q * T + i;
+        return (i < WIN && b + 1 < n) ? xn[t] : __builtin_inf();
     }
 };
+
+// Inclusive prefix sum over the 64 lanes of a wave (lane i gets the sum of
+// lanes 0..i): shifts by 1, 2, 4, 8 within rows of 16 lanes, then each row's
+// total into the rows above it.
+__device__ __forceinline__ double wave_prefix(double v)
+{
+    v += dpp_move<0x111, 0xF>(v);   // row_shr:1
+    v += dpp_move<0x112, 0xF>(v);   // row_shr:2
+    v += dpp_move<0x114, 0xF>(v);   // row_shr:4
+    v += dpp_move<0x118, 0xF>(v);   // row_shr:8
+    v += dpp_move<0x142, 0xA>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp_move<0x143, 0xC>(v);   // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// sk[r][c] = sum over the block's columns c' >= c of B_r(c') (sk[r][T] = 0),
+// by one wave: lane l of group g takes column T - 1 - (l + 64 g), so a prefix
+// over the lanes is a suffix over the columns, and the groups before g hold
+// the columns above.
+template <int KT, int U, int NW>
+__device__ __forceinline__ void strip_suffix_sums(const AggArgs& a, StripSmem<KT, U, NW>& sm,
+                                                  const int k0, const int lane)
+{
+    constexpr int R = Kern<KT>::R;
+    constexpr int T = StripSmem<KT, U, NW>::T;
+    double carry[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) carry[r] = 0.0;
+#pragma unroll
+    for (int g = 0; g < U; ++g) {
+        const int c = T - 1 - (lane + kWave * g);
+        double f[R];
+        Kern<KT>::col(f, sm.xw[c], sm.colc[c], sm.colic[c]);
+        const double Nk = k0 + c < a.n ? sm.coln[c] : 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const double p = wave_prefix(k0 + c < a.n ? Nk * f[r] : 0.0);
+            sm.sk[r][c] = p + carry[r];
+            carry[r] += lane_value(p, kWave - 1);
+        }
+    }
+    if (lane < R) sm.sk[lane][T] = 0.0;
+}
 
 // One visit of column block q by the workgroup: chunks [c_lo, c_hi).  On
 // entry `cols` holds block q's column data and the LDS window is zero; on exit
 // `cols` holds block next_q's (if next_q >= 0, loaded before the window's
 // global atomics so those loads do not wait behind them) and this thread's
 // window bins are flushed and zero again.
+//
+// Rows are split over the waves by cost (a near row costs more than a far
+// one).  Each wave takes its rows in pieces of up to 128; a piece's leading
+// rows that are far from every column of the block run the far-pair loop,
+// the rest the near-pair path.  Deaths: a row below the block loses
+// sum_r A_r(j) TK_r (TK_r: the block's column sums); a row of the diagonal
+// tile, sum_r A_r(j) sk_r(j + 1) (the columns above it); the near path adds
+// each column's losses to its pairs explicitly, the diagonal pair's full
+// rate included; the far loop nets a column's losses into its births.
 template <int KT, int U, int NW = StripWaves<U>::value>
-__device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, const int c_lo,
+__device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const StripPlan& plan,
+                                                const int q, const int c_lo,
                                                 const int c_hi, const int next_q,
                                                 StripCols<KT, U, NW>& cols,
                                                 StripSmem<KT, U, NW>& sm)
@@ -238,15 +377,17 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
     for (int t = 0; t < Cols::NW; ++t) {
         const int i = tid + NTH * t;
         if (i <= WIN) sm.xw[i] = cols.pivot(n, q, t);
+        // 1/(x_{i+1} - x_i) of the window bins (0 from bin n-1 on)
+        if (i < WIN)
+            sm.idw[i] = k0 + i < n - 1 ? 1.0 / (cols.next_pivot(n, q, t) - cols.pivot(n, q, t))
+                                       : 0.0;
     }
-    // the block's columns: 1/(x_{k+1} - x_k) (the near path forms the rest of
-    // the window's when it needs them), populations, cube roots
+    // the block's columns: populations, cube roots
 #pragma unroll
     for (int t = 0; t < Cols::NT; ++t) {
         const int i = tid + NTH * t, k = k0 + i;
         if (i < T) {
             const double xk_ = cols.pivot(n, q, t);
-            sm.idw[i] = k < n - 1 ? 1.0 / (cols.next_pivot(n, q, t) - xk_) : 0.0;
             sm.coln[i] = cols.pop(n, q, t);
             double cb = 0.0, icb = 0.0;
             if (k < n) pivot_roots<KT>(xk_, cb, icb);
@@ -255,6 +396,13 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
         }
     }
     __syncthreads();
+    // a visit holding diagonal-tile rows needs the column suffix sums
+    const int rmax = min(n, k0 + T);   // rows j <= last column of the block
+    const int vr0 = c_lo * kStripRows, vr1 = min(c_hi * kStripRows, rmax);
+    if (plan.band && vr1 > k0) {
+        if (wv == 0) strip_suffix_sums<KT, U, NW>(a, sm, k0, lane);
+        __syncthreads();
+    }
 
     double xk[U];
 #pragma unroll
@@ -283,24 +431,25 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
         for (int r = 0; r < R; ++r) acc[u][r] = 0.0;
     double* myrows = sm.rowbuf[wv];
     const BirthSink<WIN> sink{sm.colwin, a.rhs, k0};
-    const int rmax = min(n, k0 + T);   // rows j <= last column of the block
     wave_lds_sync();
 
-    // The visit's rows are split evenly over the waves; a wave takes its rows
-    // in pieces of up to 128, two rows per lane, whose factor computations
+    // This wave's rows: an equal share of the visit's cost.  It takes them in
+    // pieces of up to 128, two rows per lane, whose factor computations
     // (cube-root chains) run interleaved.  Every 32 rows a wave posts its
-    // progress in LDS and sets its issue priority from its lead over the
-    // workgroup's mean (ahead: low, behind: high): the SIMD otherwise favours
-    // its oldest wave, which would finish first while the younger one on the
-    // same SIMD lags (measured: 20 us against 30 us) and the workgroup waits
-    // for it at the end of the visit.  This lane's rows of the next piece are loaded one piece
-    // ahead, before the current piece issues its global atomics: a load
-    // issued after an atomic would wait for it (vector memory counts retire
-    // in order).
+    // progress (cost done) in LDS and sets its issue priority from its lead
+    // over the workgroup's mean (ahead: low, behind: high): the SIMD
+    // otherwise favours its oldest wave, which would finish first while the
+    // younger one on the same SIMD lags (measured: 20 us against 30 us) and
+    // the workgroup waits for it at the end of the visit.  This lane's rows
+    // of the next piece are loaded one piece ahead, before the current piece
+    // issues its global atomics: a load issued after an atomic would wait for
+    // it (vector memory counts retire in order).
     constexpr int PR = kStripPieceRows / kWave;   // rows per lane in a piece
-    const int vr0 = c_lo * kStripRows, vr1 = min(c_hi * kStripRows, rmax);
-    const int w_lo = vr0 + (int)((long long)(vr1 - vr0) * wv / Cols::NWV);
-    const int w_hi = vr0 + (int)((long long)(vr1 - vr0) * (wv + 1) / Cols::NWV);
+    const VisitRows vrows(plan, q, vr0, vr1);
+    const long long vcost = vrows.total();
+    const int w_lo = vrows.row_at(vcost * wv / Cols::NWV);
+    const int w_hi = vrows.row_at(vcost * (wv + 1) / Cols::NWV);
+    const long long w_cost0 = vrows.cost(w_lo);
     // The loads are unconditional (the index clamped into the arrays; rows
     // past the piece are masked where they are used): loads under a branch
     // make the compiler wait for them at once, which undoes the prefetch.
@@ -312,20 +461,21 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
             nr[t] = a.N[j];
         }
     };
-    auto pace = [&](int done) {
+    auto pace = [&](int row) {
+        const int done = (int)(vrows.cost(row) - w_cost0);
         if (lane == 0) sm.prog[wv] = done;
         wave_lds_sync();
         int tot = 0;
 #pragma unroll
         for (int w = 0; w < Cols::NWV; ++w) tot += sm.prog[w];
-        const int lead = done * Cols::NWV - tot;   // NWV x (own rows - mean rows)
-        constexpr int L = 16 * Cols::NWV;
+        const int lead = done * Cols::NWV - tot;   // NWV x (own cost - mean cost)
+        const int L = 16 * Cols::NWV * plan.cf;    // 16 far rows
         set_wave_priority(lead > L ? 0 : lead > 0 ? 1 : lead > -L ? 2 : 3);
     };
     double xj_next[PR], Nj_next[PR];
     load_rows(w_lo, xj_next, Nj_next);
     for (int rw0 = w_lo; rw0 < w_hi; rw0 += kStripPieceRows) {
-        pace(rw0 - w_lo);
+        pace(rw0);
         const int rw1 = min(rw0 + kStripPieceRows, w_hi);
         double xj[PR], Nj[PR];
 #pragma unroll
@@ -352,13 +502,19 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
         for (int t = 0; t < PR; ++t) {
             const int j = rw0 + lane + kWave * t;
             if (j < rw1) {
-                // death of row j against the block's columns (separable),
-                // issued after the next piece's loads, so the pair loop
-                // below hides its latency
+                // death of row j against the block's columns above it
+                // (separable), issued after the next piece's loads, so the
+                // pair loop below hides its latency
                 double dth = 0.0;
+                if (j < k0) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) dth = fma(av[t][r], TK[r], dth);
-                global_add(&a.rhs[j], -dth);
+                    for (int r = 0; r < R; ++r) dth = fma(av[t][r], TK[r], dth);
+                    global_add(&a.rhs[j], -dth);
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) dth = fma(av[t][r], sm.sk[r][j - k0 + 1], dth);
+                    lds_add(&sm.colwin[j - k0], -dth);
+                }
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) myrows[(lane + kWave * t) * RS + r] = av[t][r];
@@ -367,25 +523,41 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
         wave_lds_sync();
 
         const int nrows = rw1 - rw0;
-        const int tl = (nrows - 1) / kWave;   // the piece's last row: lane (nrows-1) % 64, row tl
-        double xlast_row = xj[0];
+        // The piece's leading rows below the block that are far from every
+        // column (x increases, so they are a prefix): nf of them.
+        auto row_far = [&](int jj) {
+            const double xr = myrows[jj * RS + R];
+            bool ok = true;
 #pragma unroll
-        for (int t = 1; t < PR; ++t) xlast_row = tl == t ? xj[t] : xlast_row;
-        const double xjmax = __shfl(xlast_row, (nrows - 1) % kWave, kWave);
-        bool ok = true;   // strip rows lie below the band: never on the diagonal
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int k = k0 + lane + kWave * u;
-            ok = ok && (k >= n - 1 || (xjmax + xk[u]) < sm.xw[lane + kWave * u + 1]);
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + lane + kWave * u;
+                ok = ok && (k >= n - 1 || (xr + xk[u]) < sm.xw[lane + kWave * u + 1]);
+            }
+            return __all(ok);
+        };
+        const int nlow = min(nrows, max(0, k0 - rw0));
+        int nf;
+        if (nlow == 0) {
+            nf = 0;
+        } else if (row_far(nlow - 1)) {
+            nf = nlow;
+        } else {
+            int lo = 0, hi = nlow - 1;   // rows [0, lo) far; row hi is not
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (row_far(mid)) lo = mid + 1; else hi = mid;
+            }
+            nf = lo;
         }
-        if (__all(ok)) {
-            // every pair lands in (k, k+1), or clips at column n-1 (taken as
-            // weight w = 0 there: 1/(x_{k+1} - x_k) is 0 for k = n-1)
-            // Rows in pairs, the second row's update of an accumulator U*R
-            // instructions after the first's.  The LDS reads of the next
-            // pair are issued before the FMAs of this one, alternating two
-            // register sets, so their latency hides behind the FMAs (past
-            // the last row they read the wave's two spare rows).
+        if (nf > 0) {
+            // rows [0, nf): every pair lands in (k, k+1), or clips at column
+            // n-1 (taken as weight w = 0 there: 1/(x_{k+1} - x_k) is 0 for
+            // k = n-1).  Rows in pairs, the second row's update of an
+            // accumulator U*R instructions after the first's.  The LDS reads
+            // of the next pair are issued before the FMAs of this one,
+            // alternating two register sets, so their latency hides behind
+            // the FMAs (past the last row they read the wave's two spare
+            // rows).
             auto ld2 = [&](int jr, double (&p0)[R + 1], double (&p1)[R + 1]) {
                 const double* r0 = myrows + jr * RS;
 #pragma unroll
@@ -413,8 +585,8 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
             double pa0[R + 1], pa1[R + 1], pb0[R + 1], pb1[R + 1];
             ld2(0, pa0, pa1);
             int jj = 0;
-            for (; jj + 3 < nrows; jj += 4) {
-                if ((jj & 31) == 28) pace(rw0 - w_lo + jj);
+            for (; jj + 3 < nf; jj += 4) {
+                if ((jj & 31) == 28) pace(rw0 + jj);
                 ld2(jj + 2, pb0, pb1);
                 __builtin_amdgcn_sched_barrier(0);
                 fma2(pa0, pa1);
@@ -423,11 +595,11 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
                 fma2(pb0, pb1);
             }
             // 0 to 3 rows left; rows jj and jj + 1 are in pa
-            if (jj + 1 < nrows) {
+            if (jj + 1 < nf) {
                 fma2(pa0, pa1);
                 jj += 2;
             }
-            if (jj < nrows) {
+            if (jj < nf) {
                 const double* r0 = myrows + jj * RS;
                 const double x0 = r0[R];
 #pragma unroll
@@ -437,72 +609,122 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const int q, c
                     for (int r = 0; r < R; ++r) acc[u][r] = fma(r0[r], w0, acc[u][r]);
                 }
             }
-        } else {
-            // Pairs near the diagonal band: each pair adds its two
-            // Kumar-Ramkrishna parts to the LDS window (global atomics
-            // beyond it).  A lane keeps the exact bracket (cur, cur+1) of
-            // v = x_j + x_k for its column; rows run upward, so it only moves
-            // up, mostly by at most one bin per row.  One column at a time
-            // keeps a single bracket live.
+        }
+        if (nf < nrows) {
+            // Near pairs (rows [nf, nrows), the diagonal tile's among them):
+            // a lane keeps the exact bracket (cur, cur+1) of v = x_j + x_k
+            // for its column; rows run upward, so it only moves up.  The
+            // pairs of one bracket are summed in registers (their rate and
+            // weighted rate) and added to the LDS window (global atomics
+            // beyond it) when the bracket moves.  One column at a time keeps
+            // a single bracket live.
             auto xat = [&](int i) { return (i - k0 <= WIN) ? lds_ld(&sm.xw[i - k0]) : glb_ld(&a.x[i]); };
             auto idat = [&](int i) {
                 const int o = i - k0;
-                return o < T ? lds_ld(&sm.idw[o])
-                     : (o < WIN ? 1.0 / (lds_ld(&sm.xw[o + 1]) - lds_ld(&sm.xw[o]))
-                                : 1.0 / (glb_ld(&a.x[i + 1]) - glb_ld(&a.x[i])));
+                return o < WIN ? lds_ld(&sm.idw[o]) : 1.0 / (glb_ld(&a.x[i + 1]) - glb_ld(&a.x[i]));
             };
-            double Bc[U][R];
-            strip_col_factors<KT, U, NW>(a, sm, k0, lane, Bc);
-#pragma unroll
+            // one loop body for the U columns (not unrolled: this path's
+            // code stays small next to the far loop's)
+#pragma unroll 1
             for (int u = 0; u < U; ++u) {
-                const int k = k0 + lane + kWave * u;
-                if (k >= n) continue;
-                int cur = k;
-                double xc = xk[u], xc1 = k < n - 1 ? sm.xw[lane + kWave * u + 1] : xk[u];
-                double idc = sm.idw[lane + kWave * u];
-                double accc = 0.0, dk = 0.0;
-                const int jend = min(nrows, k - rw0 + 1);   // rows j <= k
-                for (int jj = 0; jj < jend; ++jj) {
-                    const double* rp = myrows + jj * RS;
-                    double pr = rp[0] * Bc[u][0];
+                const int c = lane + kWave * u;
+                const int k = k0 + c;
+                const int jend = k < n ? min(nrows, k - rw0 + 1) : 0;   // rows j <= k
+                if (!__any(jend > nf)) continue;
+                // this column's factors B_r(k)
+                const double xku = sm.xw[c];
+                double Bu[R];
+                {
+                    double f[R];
+                    K::col(f, xku, sm.colc[c], sm.colic[c]);
+                    const double Nk = k < n ? sm.coln[c] : 0.0;
 #pragma unroll
-                    for (int r = 1; r < R; ++r) pr = fma(rp[r], Bc[u][r], pr);
-                    if (rw0 + jj == k) pr *= 0.5;
-                    dk += pr;
-                    const double v = rp[R] + xk[u];
+                    for (int r = 0; r < R; ++r) Bu[r] = Nk * f[r];
+                }
+                // bracket (cur, cur + 1) with its pivots and 1/(x_{cur+1} -
+                // x_cur), and the next bracket's pivot x_{cur+2} and
+                // reciprocal, so that a move by one bin waits for no load
+                int cur = k;
+                double xc = xku, xc1 = sm.xw[c + 1], idc = sm.idw[c];
+                double xc2 = k + 2 < n ? xat(k + 2) : __builtin_inf();
+                double idn = k + 1 < n - 1 ? idat(k + 1) : 0.0;
+                double accr = 0.0, accw = 0.0, accc = 0.0, dk = 0.0;
+                // the rows' factors, the next row's loaded before this one's use
+                double rn[R + 1];
+                {
+                    const double* r0 = myrows + nf * RS;
+#pragma unroll
+                    for (int r = 0; r <= R; ++r) rn[r] = r0[r];
+                }
+                for (int jj = nf; jj < jend; ++jj) {
+                    double rw[R + 1];
+#pragma unroll
+                    for (int r = 0; r <= R; ++r) rw[r] = rn[r];
+                    {
+                        const double* r1 = myrows + (jj + 1) * RS;   // spare row past the end
+#pragma unroll
+                        for (int r = 0; r <= R; ++r) rn[r] = r1[r];
+                    }
+                    double rf = rw[0] * Bu[0];
+#pragma unroll
+                    for (int r = 1; r < R; ++r) rf = fma(rw[r], Bu[r], rf);
+                    const bool dg = rw0 + jj == k;
+                    const double pr = dg ? 0.5 * rf : rf;
+                    dk += rf;   // column k's loss: the diagonal pair's 2 x rate/2 too
+                    const double v = rw[R] + xku;
                     if (v >= xlast) {
                         // upper clip (src/aggregation_kernels.cuh:191-192)
                         accc += pr;
-                    } else {
-                        if (v >= xc1) {
-                            int cn = cur + 1;
-                            if (xat(cn + 1) <= v) {
-                                const int hi = (a.inv_log_r != 0.0)
+                        continue;
+                    }
+                    if (v >= xc1) {
+                        if (accr != 0.0 || accw != 0.0) {
+                            sink.put(cur, accr - accw);
+                            sink.put(cur + 1, accw);
+                        }
+                        if (v < xc2) {
+                            cur += 1;
+                            xc = xc1;
+                            xc1 = xc2;
+                            idc = idn;
+                        } else {
+                            int cn;
+                            if (v < lds_ld(&sm.xw[WIN])) {
+                                cn = k0 + window_bracket_lo(sm.xw, cur + 2 - k0, WIN, v);
+                            } else {
+                                const int hb = (a.inv_log_r != 0.0)
                                     ? bracket_geometric(a.x, n, v, a.log_x0, a.inv_log_r)
-                                    : bracket_search(a.x, cn, n - 1, v);
-                                cn = hi - 1;
+                                    : bracket_search(a.x, cur + 2, n - 1, v);
+                                cn = hb - 1;
                             }
                             cur = cn;
                             xc = xat(cn);
                             xc1 = xat(cn + 1);
                             idc = idat(cn);
                         }
-                        const double wp = ((v - xc) * idc) * pr;
-                        sink.put(cur, pr - wp);
-                        sink.put(cur + 1, wp);
+                        xc2 = cur + 2 < n ? xat(cur + 2) : __builtin_inf();
+                        idn = cur + 1 < n - 1 ? idat(cur + 1) : 0.0;
+                        accr = 0.0;
+                        accw = 0.0;
                     }
+                    accr += pr;
+                    accw = fma((v - xc) * idc, pr, accw);
+                }
+                if (accr != 0.0 || accw != 0.0) {
+                    sink.put(cur, accr - accw);
+                    sink.put(cur + 1, accw);
                 }
                 if (accc != 0.0) sink.put(n - 1, accc);
-                if (dk != 0.0) sink.put(k, -dk);   // column k's deaths
+                if (dk != 0.0) sink.put(k, -dk);   // column k's losses
             }
         }
         // the next piece rewrites this wave's rows
         wave_lds_sync();
     }
 
-    // ---- far pairs of this wave's chunks: a far pair (j, k) adds
-    // rate (1 - w) to bin k, rate w to bin k+1 and takes rate from bin k, so
-    // bin k nets -rate w and bin k+1 +rate w ----
+    // ---- far pairs of this wave's rows: a far pair (j, k) adds rate (1 - w)
+    // to bin k, rate w to bin k+1 and takes rate from bin k, so bin k nets
+    // -rate w and bin k+1 +rate w ----
     double Bc[U][R];
     strip_col_factors<KT, U, NW>(a, sm, k0, lane, Bc);
 #pragma unroll
