@@ -75,32 +75,40 @@ def build_cpp_tests():
 
 
 def build_host_checks():
-    """tests/cpp/test_roots.cpp: host-side check of the device cube root."""
-    src = os.path.join(REPO, "tests", "cpp", "test_roots.cpp")
-    out = os.path.join(REPO, "tests", "cpp", "test_roots")
-    _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", f"-I{CSRC}", src, "-o", out + ".tmp"])
-    os.replace(out + ".tmp", out)
-    return out
+    """tests/cpp/test_roots.cpp: host-side check of the device cube root;
+    tests/cpp/test_strip_plan.cpp: host-side check of the strip launch's
+    work split."""
+    for name in ("test_roots", "test_strip_plan"):
+        src = os.path.join(REPO, "tests", "cpp", name + ".cpp")
+        out = os.path.join(REPO, "tests", "cpp", name)
+        _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", f"-I{CSRC}", src, "-o", out + ".tmp"])
+        os.replace(out + ".tmp", out)
 
 
 def build_bench_helpers():
-    """bench/batched_loop.cpp: batched launch vs a loop of plain calls;
-    bench/fp64_issue.hip: FP64 FMA issue rates (waves per SIMD x chains);
-    bench/atomic_rates.hip: LDS and global FP64 atomic-add rates."""
+    """bench/batched_loop.cpp: batched launch vs a loop of plain calls (the
+    bench's batched form)."""
     src = os.path.join(HERE, "bench", "batched_loop.cpp")
     out = os.path.join(HERE, "bench", "batched_loop")
     _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", src, "-o", out + ".tmp",
           f"-L{LIBDIR}", "-lpbe_hip", "-Wl,-rpath,$ORIGIN/../lib"])
     os.replace(out + ".tmp", out)
-    src = os.path.join(HERE, "bench", "fp64_issue.hip")
-    fp = os.path.join(HERE, "bench", "fp64_issue")
-    _run([HIPCC, *COMMON, src, "-o", fp + ".tmp"])
-    os.replace(fp + ".tmp", fp)
-    src = os.path.join(HERE, "bench", "atomic_rates.hip")
-    ar = os.path.join(HERE, "bench", "atomic_rates")
-    _run([HIPCC, *COMMON, src, "-o", ar + ".tmp"])
-    os.replace(ar + ".tmp", ar)
     return out
+
+
+def build_tools():
+    """Measurement tools, built on request (build_lib.py --tools), not by
+    build(): bench/atomic_rates.hip (LDS and global FP64 atomic-add rates)
+    and bench/fp64_issue.hip (FP64 FMA issue rates by waves per SIMD and
+    independent chains per wave)."""
+    outs = []
+    for name in ("atomic_rates", "fp64_issue"):
+        src = os.path.join(HERE, "bench", name + ".hip")
+        exe = os.path.join(HERE, "bench", name)
+        _run([HIPCC, *COMMON, src, "-o", exe + ".tmp"])
+        os.replace(exe + ".tmp", exe)
+        outs.append(exe)
+    return outs
 
 
 def build(verbose=False):
@@ -113,4 +121,6 @@ def build(verbose=False):
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    if "--tools" in sys.argv[1:]:
+        print(build_tools())
     sys.exit(0)

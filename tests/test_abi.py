@@ -133,6 +133,21 @@ def test_device_cube_root_host_check():
     assert out["cbrt_max_ulp"] <= 4 and out["inv_cbrt_max_ulp"] <= 8
 
 
+def test_strip_plan_host_check():
+    """The strip launch's work split (agg_strip.hpp), compiled for the host:
+    every chunk of every column block goes to exactly one workgroup share,
+    the cost line equals the sum of its chunk costs, and the waves' row
+    ranges tile each visit (tests/cpp/test_strip_plan.cpp)."""
+    import json
+    import subprocess
+    exe = os.path.join(REPO, "tests", "cpp", "test_strip_plan")
+    assert os.path.exists(exe), "tests/cpp/test_strip_plan not built (run build())"
+    r = subprocess.run([exe], stdout=subprocess.PIPE, text=True, timeout=120)
+    out = json.loads(r.stdout)
+    assert r.returncode == 0 and out["failures"] == 0, out
+    assert out["chunks"] > 1e7
+
+
 def test_deterministic_argument_errors(pbe):
     """The gather-mode launches check arguments, then the workspace size,
     before any HIP call."""
