@@ -305,6 +305,7 @@ static StripPlan make_strip_plan(int n, int U, bool band, double inv_log_r)
         const double reach = -std::log(std::expm1(1.0 / inv_log_r)) * inv_log_r;
         if (reach > T && reach < 1e7) plan.m_near = (int)std::ceil((reach - T) / kStripRows);
     }
+    plan.mq = plan.m_near / U + 1;
     plan.cf = 4;
     plan.cn = 40;
     plan.cd = 6;

@@ -55,6 +55,7 @@ struct StripPlan {
     int U;             // chunks per block height (T / 64)
     int band;          // 1: blocks hold their diagonal band rows too
     int m_near;        // strip chunks below the band charged as near
+    int mq;            // m_near / U + 1 (formed on the host)
     int cf, cn, cd, cv;
 };
 
@@ -92,7 +93,7 @@ __host__ __device__ __forceinline__ long long strip_se_sum(const StripPlan& s, i
 __host__ __device__ __forceinline__ long long strip_sf_sum(const StripPlan& s, int q)
 {
     // sum over a = q' - 1 in [0, q - 2] of max(0, U a - m_near)
-    const long long A = q - 2, a0 = s.m_near / s.U + 1;
+    const long long A = q - 2, a0 = s.mq;
     if (A < a0) return 0;
     return (long long)s.U * (A * (A + 1) / 2 - a0 * (a0 - 1) / 2) -
            (long long)s.m_near * (A - a0 + 1);
@@ -115,9 +116,49 @@ __host__ __device__ __forceinline__ long long strip_cost_total(const StripPlan& 
     return strip_cost_start(s, s.nb);
 }
 
+// ceil(a / b) for 0 <= a < 2^53 and 0 < b <= 2^20, through a double
+// quotient: a / b lies at least 1/b from any integer it does not equal, far
+// more than the quotient's rounding, so the result is exact (a 64-bit
+// integer division is a long software sequence on the GPU).
 __host__ __device__ __forceinline__ long long ceil_div(long long a, long long b)
 {
-    return (a + b - 1) / b;
+    return (long long)ceil((double)a / (double)b);
+}
+
+// Cost of block q's visit and chunks.
+__host__ __device__ __forceinline__ long long strip_block_cost(const StripPlan& s, int q)
+{
+    const int nc = strip_nc(s, q), se = strip_se(s, q), sf = strip_sf(s, q);
+    return (nc > 0 ? s.cv : 0) + (long long)s.cf * sf + (long long)s.cn * (se - sf) +
+           (long long)s.cd * (nc - se);
+}
+
+// Block whose cost range holds f (largest q < nb with cost_start(q) <= f).
+// Past the first few blocks a block costs A + B q with B = cf U, so the
+// start of block q is close to A q + B q (q - 1) / 2: a square root gives
+// the block to within a step or two, then exact comparisons settle it.
+__host__ __device__ __forceinline__ int strip_block_at_cost(const StripPlan& s, long long f)
+{
+    const double B = (double)s.cf * s.U;
+    const int qr = s.nb - 1;
+    const double A = (double)strip_block_cost(s, qr) - B * qr;
+    const double b = A - 0.5 * B;
+    double g = (sqrt(b * b + 2.0 * B * (double)f) - b) / B;
+    int q = g > 0.0 ? (int)g : 0;
+    if (q > qr) q = qr;
+    int steps = 0;
+    while (q > 0 && strip_cost_start(s, q) > f && ++steps < 8) --q;
+    while (q < qr && strip_cost_start(s, q + 1) <= f && ++steps < 16) ++q;
+    if (steps >= 8) {
+        // far from the estimate (small problems): bisect
+        int lo = 0, hi = qr;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (strip_cost_start(s, mid) <= f) lo = mid; else hi = mid - 1;
+        }
+        q = lo;
+    }
+    return q;
 }
 
 // First chunk (q, c) whose start on the cost line is at or after f;
@@ -131,12 +172,7 @@ __host__ __device__ __forceinline__ void strip_chunk_at_cost(const StripPlan& s,
         c = 0;
         return;
     }
-    int lo = 0, hi = s.nb - 1;   // largest block with cost_start <= f
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (strip_cost_start(s, mid) <= f) lo = mid; else hi = mid - 1;
-    }
-    q = lo;
+    q = strip_block_at_cost(s, f);
     const int nc = strip_nc(s, q), se = strip_se(s, q), sf = strip_sf(s, q);
     const long long off = f - strip_cost_start(s, q) - (nc > 0 ? s.cv : 0);
     long long cc;
@@ -160,9 +196,9 @@ __host__ __device__ __forceinline__ void strip_chunk_at_cost(const StripPlan& s,
 // cn below row 64 se and cd above.  row_at(t) is the first row whose cost
 // offset from r0 is at least t; cost(j) the offset of row j.
 struct VisitRows {
-    int r0, r1, b1, b2;
-    long long wf, wn, wd, c1, c2;
+    int r0, r1, b1, b2, wf, wn, wd, c1, c2;
     __host__ __device__ static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+    __host__ __device__ static int ceil_i(int a, int b) { return (int)ceil((double)a / (double)b); }
     __host__ __device__ __forceinline__ VisitRows(const StripPlan& s, int q, int r0_, int r1_)
     {
         r0 = r0_;
@@ -175,18 +211,18 @@ struct VisitRows {
         c1 = wf * (b1 - r0);
         c2 = c1 + wn * (b2 - b1);
     }
-    __host__ __device__ __forceinline__ long long total() const { return c2 + wd * (r1 - b2); }
-    __host__ __device__ __forceinline__ long long cost(int j) const
+    __host__ __device__ __forceinline__ int total() const { return c2 + wd * (r1 - b2); }
+    __host__ __device__ __forceinline__ int cost(int j) const
     {
         return j <= b1 ? wf * (j - r0) : (j <= b2 ? c1 + wn * (j - b1) : c2 + wd * (j - b2));
     }
-    __host__ __device__ __forceinline__ int row_at(long long t) const
+    __host__ __device__ __forceinline__ int row_at(int t) const
     {
-        long long j;
-        if (t <= c1) j = r0 + ceil_div(t, wf);
-        else if (t <= c2) j = b1 + ceil_div(t - c1, wn);
-        else j = b2 + ceil_div(t - c2, wd);
-        return (int)(j < r0 ? r0 : (j > r1 ? r1 : j));
+        int j;
+        if (t <= c1) j = r0 + ceil_i(t, wf);
+        else if (t <= c2) j = b1 + ceil_i(t - c1, wn);
+        else j = b2 + ceil_i(t - c2, wd);
+        return clampi(j, r0, r1);
     }
 };
 
@@ -445,11 +481,16 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const StripPla
     // issues its global atomics: a load issued after an atomic would wait for
     // it (vector memory counts retire in order).
     constexpr int PR = kStripPieceRows / kWave;   // rows per lane in a piece
+    // (a visit of far rows only splits them evenly: the same split, found
+    // with less arithmetic)
     const VisitRows vrows(plan, q, vr0, vr1);
-    const long long vcost = vrows.total();
-    const int w_lo = vrows.row_at(vcost * wv / Cols::NWV);
-    const int w_hi = vrows.row_at(vcost * (wv + 1) / Cols::NWV);
-    const long long w_cost0 = vrows.cost(w_lo);
+    const bool far_only = vrows.b1 == vr1;
+    const int vcost = vrows.total();
+    const int w_lo = far_only ? vr0 + (vr1 - vr0) * wv / Cols::NWV
+                              : vrows.row_at(vcost * wv / Cols::NWV);
+    const int w_hi = far_only ? vr0 + (vr1 - vr0) * (wv + 1) / Cols::NWV
+                              : vrows.row_at(vcost * (wv + 1) / Cols::NWV);
+    const int w_cost0 = vrows.cost(w_lo);
     // The loads are unconditional (the index clamped into the arrays; rows
     // past the piece are masked where they are used): loads under a branch
     // make the compiler wait for them at once, which undoes the prefetch.
@@ -462,7 +503,7 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const StripPla
         }
     };
     auto pace = [&](int row) {
-        const int done = (int)(vrows.cost(row) - w_cost0);
+        const int done = far_only ? (row - w_lo) * plan.cf : vrows.cost(row) - w_cost0;
         if (lane == 0) sm.prog[wv] = done;
         wave_lds_sync();
         int tot = 0;

@@ -40,6 +40,7 @@ int main()
                         s.nb = (n + 64 * U - 1) / (64 * U);
                         s.band = band;
                         s.m_near = m_near;
+                        s.mq = m_near / U + 1;
                         s.cf = 4;
                         s.cn = 10;
                         s.cd = 6;
@@ -81,7 +82,7 @@ int main()
                                 const int r0 = c * 64, r1 = hi * 64 < rmax ? hi * 64 : rmax;
                                 if (r0 >= r1) continue;
                                 const VisitRows vr(s, q, r0, r1);
-                                const long long vc = vr.total();
+                                const int vc = vr.total();
                                 int prev = r0;
                                 for (int w = 0; w < 8; ++w) {
                                     const int lo = vr.row_at(vc * w / 8), hi2 = vr.row_at(vc * (w + 1) / 8);
