@@ -206,6 +206,31 @@ def deterministic_form(pbe_hip, hip, P, n=16384, steps=10):
             "pair_updates_per_s": P.pair_units(n, nq) / (ms * 1e-3)}
 
 
+def flocculation_form():
+    """The reference's two-stage flocculation case study end to end
+    (examples/flocculation_two_stage, 3600 RK4 steps, G 100 -> 20 1/s, the
+    time loop on the device), at N = 960 and N = 3840: wall time of the
+    program's loop (its 60 prints with host copies included, as in the
+    reference's) beside the reference's published A30 figures."""
+    import subprocess
+    exe = os.path.join(REPO, "pbe-fixed-pivot-cuda_amd", "examples", "flocculation_two_stage",
+                       "example_flocculation_two_stage")
+    published = {960: ("~4 s on an NVIDIA A30 (examples/flocculation_two_stage/main.cpp:54)", 4.0),
+                 3840: ("37.6 s on an NVIDIA A30 (README.md:43)", 37.6)}
+    out = []
+    for n in (960, 3840):
+        r = subprocess.run([exe, str(n)], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                           text=True, timeout=240)
+        if r.returncode != 0:
+            out.append({"n": n, "error": r.stderr[-300:]})
+            continue
+        d = json.loads(r.stdout.strip().splitlines()[-1])
+        d["reference_published"], ref_s = published[n]
+        d["speedup_vs_published"] = ref_s / d["wall_s"]
+        out.append(d)
+    return out
+
+
 def spot_check(got, N, x, agg, brk, t_q, bw_q):
     """Every entry of one GPU step against the OpenMP port, within
     1e-10 max|port| + 1e-9 |port_i| (the port sums in plain FP64 in another
@@ -402,6 +427,7 @@ def main():
                         "deterministic": deterministic_form(pbe_hip, hip, P, n)}
         out["forms"]["deterministic"]["over_atomic"] = (
             out["forms"]["deterministic"]["ms_per_step"] / ms_step)
+        out["forms"]["flocculation_two_stage"] = flocculation_form()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, x, agg, brk, t_q, bw_q, args.cpu_seconds)
     print(json.dumps(out))

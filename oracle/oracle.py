@@ -136,14 +136,15 @@ class _Problem(ctypes.Structure):
         ("n_quad", ctypes.c_int), ("t_q", _dp), ("bw_q", _dp),
         ("selection", ctypes.c_int), ("S0", ctypes.c_double),
         ("v_ref", ctypes.c_double), ("alpha", ctypes.c_double),
-        ("v_min", ctypes.c_double),
+        ("v_min", ctypes.c_double), ("threads", ctypes.c_int),
     ]
 
 
 def rk4(N0, x, dt, steps, agg=None, brk=None, t_q=None, bw_q=None,
-        bracket=BRACKET_EXACT):
+        bracket=BRACKET_EXACT, threads=1):
     """Classic RK4 of the reference harness; agg/brk are dicts of parameters
-    (keys as in aggregation_rhs / breakage_rhs)."""
+    (keys as in aggregation_rhs / breakage_rhs); threads > 1 splits each
+    aggregation evaluation over OpenMP threads."""
     N = _f64(N0).copy()
     x = _f64(x)
     keep = [x]
@@ -151,6 +152,7 @@ def rk4(N0, x, dt, steps, agg=None, brk=None, t_q=None, bw_q=None,
     pr.n = len(N)
     pr.x = _p(x)
     pr.bracket_mode = bracket
+    pr.threads = int(threads)
     if agg:
         pr.use_agg = 1
         pr.kernel = int(agg["kernel"])

@@ -330,9 +330,10 @@ static int eval_rhs(const oracle_problem_t* p, const double* Nin, double* k)
     memset(k, 0, sizeof(double) * (size_t)p->n);
     int e = 0;
     if (p->use_agg)
-        e = oracle_aggregation_rhs(p->n, Nin, p->x, p->kernel, p->beta0,
-                                   p->beta_br, p->beta_sh, p->log_x0,
-                                   p->inv_log_r, p->bracket_mode, k, NULL, NULL);
+        e = oracle_aggregation_rhs_threads(p->n, Nin, p->x, p->kernel, p->beta0,
+                                           p->beta_br, p->beta_sh, p->log_x0,
+                                           p->inv_log_r, p->bracket_mode, k, NULL, NULL,
+                                           p->threads);
     if (e) return e;
     if (p->use_brk)
         e = oracle_breakage_rhs(p->n, p->n_quad, Nin, p->x, p->t_q, p->bw_q,

@@ -57,6 +57,7 @@ typedef struct {
     const double* bw_q;
     int selection;
     double S0, v_ref, alpha, v_min;
+    int threads;          /* OpenMP threads of the aggregation sums (<= 1: serial) */
 } oracle_problem_t;
 
 /* classic RK4, `steps` steps of size dt, in place on Nst */

@@ -96,6 +96,27 @@ def build_bench_helpers():
     return out
 
 
+EXAMPLES = ["aggregation_constant", "aggregation_sum", "aggregation_product",
+            "breakage_linear_symmetricbinary", "breakage_linear_uniform",
+            "flocculation_two_stage"]
+
+
+def build_examples():
+    """examples/<name>/main.cpp -> examples/<name>/example_<name>: the
+    reference's example programs over the drop-in headers (host code linked
+    to lib/libpbe_hip.so)."""
+    cmds, outs = [], []
+    for name in EXAMPLES:
+        src = os.path.join(HERE, "examples", name, "main.cpp")
+        out = os.path.join(HERE, "examples", name, "example_" + name)
+        cmds.append([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", src, "-o", out,
+                     f"-L{LIBDIR}", "-lpbe_hip", "-Wl,-rpath,$ORIGIN/../../lib"])
+        outs.append(out)
+    with cf.ThreadPoolExecutor(max_workers=min(len(cmds), 8)) as ex:
+        list(ex.map(_run, cmds))
+    return outs
+
+
 def build_tools():
     """Measurement tools, built on request (build_lib.py --tools), not by
     build(): bench/atomic_rates.hip (LDS and global FP64 atomic-add rates)
@@ -116,6 +137,7 @@ def build(verbose=False):
     exe = build_cpp_tests()
     build_host_checks()
     build_bench_helpers()
+    build_examples()
     return lib, exe
 
 

@@ -303,3 +303,71 @@ def log_misfit(N, logT, mask, eps):
     L = float(np.sum(mask * d * d))
     g = np.where(N > eps, 2.0 * mask * d / Nc, 0.0)
     return L, g
+
+
+# ------------------------------------------- two-stage flocculation example --
+
+FLOC_CONSTANTS = dict(kB=1.380649e-23, d0=0.5e-6, phi=5.0e-5, T=298.0, mu=1.0e-3,
+                      d_min=0.05e-6, d_max=200.0e-6, G_high=100.0, G_low=20.0, G_ref=1.0,
+                      dt_s=0.5, d_crit=5.0e-6, k_br=1.0e-3, alpha_br=2.0, eps_ero=0.05)
+
+
+def config_flocculation_two_stage(n=960, t_end_s=1800.0, t_mid_s=600.0):
+    """The reference's two-stage flocculation case study in its dimensionless
+    form (examples/flocculation_two_stage/main.cpp:97-205, 226-331): grid of
+    n geometric sections over d in [0.05, 200] um (x = v / v0), log-normal
+    start sigma_ln = 0.3 centred on x = 1 (normalised by a running sum, in
+    index order), Brownian + shear aggregation (beta_br = 0.25,
+    beta_sh = G v0 / (pi beta0)), power-law selection S0 = k_br G t_c,
+    v_ref = (d_crit / d0)^3, alpha = 2, erosion daughter (0.95, 0.05),
+    RK4 with dt = 0.5 s / t_c.  Returns a dict with x, N0, the parameter
+    dicts for each shear rate, the daughter table, dt, the step count, the
+    number of high-shear steps (a step is high-shear while the accumulated
+    dimensionless time times t_c is below t_mid, as in the reference's loop)
+    and the step of the state at t_mid."""
+    c = FLOC_CONSTANTS
+    v0 = (math.pi / 6.0) * c["d0"] * c["d0"] * c["d0"]
+    N0s = c["phi"] / v0
+    beta0 = 4.0 * (2.0 * c["kB"] * c["T"] / (3.0 * c["mu"]))
+    t_c = 1.0 / (beta0 * N0s)
+    x = geometric_grid_range(n, (c["d_min"] / c["d0"]) ** 3.0, (c["d_max"] / c["d0"]) ** 3.0)
+    lr = math.log(x[1] / x[0])
+    N = np.empty(n)
+    s = 0.0
+    for i in range(n):
+        z = math.log(x[i]) / 0.3
+        N[i] = math.exp(-0.5 * z * z) * (x[i] * lr)
+        s += N[i]
+    N = N / s
+    log_x0, inv_log_r = grid_lookup(x)
+
+    def agg(G):
+        return dict(kernel=5, beta0=0.0, beta_br=0.25, beta_sh=G * v0 / (math.pi * beta0),
+                    log_x0=log_x0, inv_log_r=inv_log_r)
+
+    def brk(G):
+        return dict(selection=2, S0=c["k_br"] * G / c["G_ref"] * t_c,
+                    v_ref=(c["d_crit"] / c["d0"]) ** 3.0, alpha=c["alpha_br"], v_min=0.0)
+
+    dt_nd = c["dt_s"] / t_c
+    steps = int(t_end_s / c["dt_s"])
+    n_high, t_nd = 0, 0.0
+    for k in range(steps):
+        if t_nd * t_c < t_mid_s:
+            n_high = k + 1
+        t_nd += dt_nd
+    return dict(x=x, N0=N, agg_high=agg(c["G_high"]), agg_low=agg(c["G_low"]),
+                brk_high=brk(c["G_high"]), brk_low=brk(c["G_low"]),
+                t_q=np.array([1.0 - c["eps_ero"], c["eps_ero"]]), bw_q=np.array([1.0, 1.0]),
+                dt=dt_nd, steps=steps, n_high=n_high,
+                mid_step=int(t_mid_s / c["dt_s"] + 0.5), t_c=t_c, d0=c["d0"])
+
+
+def d43(N, x, d_ref):
+    """Volume-weighted mean diameter sum N d^4 / sum N d^3 of pivots x (volumes
+    in units of a reference volume of diameter d_ref), summed in index order."""
+    num = den = 0.0
+    for Ni, xi in zip(N, x):
+        num += Ni * xi ** (4.0 / 3.0)
+        den += Ni * xi
+    return d_ref * num / den if den > 0.0 else 0.0
