@@ -212,3 +212,27 @@ def test_noalloc_argument_errors(pbe):
              None) == INVALID_VALUE
     assert f(FAKE, FAKE, FAKE, ctypes.byref(pbe.AggregationParams(n=65537)), None) == \
         INVALID_VALUE
+
+
+def test_flocculation_two_stage_setup():
+    """The case study's dimensionless set-up (reference
+    examples/flocculation_two_stage/main.cpp): the log-normal start has
+    M0 = 1 and M1 = 1.1445 on its grid (the figure the reference's utilities
+    quote, examples/common/pbe_examples_utils.hpp:141); the shear rate
+    switches after the step that starts below t = 600 s by the accumulated
+    dimensionless time, which is step 1201 of 3600; the dimensionless shear
+    and selection coefficients follow from the physical constants."""
+    import math
+    from pbe_hip import problems as P
+    c = P.config_flocculation_two_stage(960)
+    assert abs(c["N0"].sum() - 1.0) < 1e-15
+    assert round(float((c["x"] * c["N0"]).sum()), 4) == 1.1445
+    assert c["steps"] == 3600 and c["n_high"] == 1201 and c["mid_step"] == 1200
+    assert abs(c["agg_high"]["beta_sh"] / c["agg_low"]["beta_sh"] - 5.0) < 1e-14
+    assert abs(c["t_c"] - 119.30821604109944) < 1e-9
+    d = P.FLOC_CONSTANTS
+    assert abs(c["x"][0] - (d["d_min"] / d["d0"]) ** 3) < 1e-18
+    assert abs(c["x"][-1] / (d["d_max"] / d["d0"]) ** 3 - 1.0) < 1e-10
+    assert c["brk_high"]["S0"] == d["k_br"] * 100.0 * c["t_c"]
+    assert math.isclose(P.d43(c["N0"], c["x"], 1.0), float((c["x"] ** (4 / 3) * c["N0"]).sum()
+                                                            / (c["x"] * c["N0"]).sum()))
