@@ -92,11 +92,11 @@ hipError_t pbe_launch_aggregation_rhs(const double* N, const double* x,
  * whose only knob is the compile-time PBE_TILE, src/aggregation_kernels.cuh:
  * 30-33).  tile_u = columns per lane: column blocks and diagonal-band tiles
  * are 64*tile_u wide; band_split = row slices per diagonal-band tile (each
- * slice one workgroup).  0 picks the library's autotuned choice for
- * (n, kernel_type); on a geometric grid (inv_log_r > 0) whose far-pair
- * reach log_r(1/(r-1)) bins exceeds 64*tile_u, tile_u = 0 takes 8 when
- * n >= 6000 and the reach is at most 512 bins (the column strips then hold
- * far pairs only), or when n >= 32768. */
+ * slice one workgroup; offset segments per block for the 256-wide band
+ * sweeps).  0 picks the library's autotuned choice for (n, kernel_type,
+ * grid reach), where the reach of a geometric grid (inv_log_r > 0) is
+ * log_r(1/(r-1)) bins: how far below the diagonal a pair can still land
+ * beyond its bracket (k, k+1). */
 typedef struct pbe_agg_options {
     int tile_u;       /* 0, 1, 2, 4 or 8 */
     int band_split;   /* 0, 1, 2, 4, 8 or 16 */
@@ -109,9 +109,11 @@ hipError_t pbe_launch_aggregation_rhs_ex(const double* N, const double* x,
                                          const pbe_agg_options* options,
                                          hipStream_t stream);
 
-/* The autotuned tile_u for (n, kernel_type): what tile_u = 0 resolves to,
- * save the widening to 8 on fine geometric grids described above. */
-int pbe_aggregation_default_tile(int n, int kernel_type);
+/* The autotuned options for params (n, kernel_type and the grid's reach):
+ * what tile_u = 0 and band_split = 0 resolve to.  hipErrorInvalidValue for
+ * NULL pointers, n <= 0 or an unknown kernel_type. */
+hipError_t pbe_aggregation_default_options(const pbe_aggregation_params* params,
+                                           pbe_agg_options* options);
 
 /*
  * rhs[0..n) += fixed-pivot breakage source (death and quadrature birth).
@@ -232,6 +234,13 @@ typedef struct pbe_batch_plan_s* pbe_batch_plan;
 hipError_t pbe_batch_plan_create(pbe_batch_plan* out,
                                  const pbe_batch_instance* instances, int count,
                                  int kernel_type, int selection, int n_quad);
+/* pbe_batch_plan_create with every instance's pair tiles 64*tile_u wide
+ * (tile_u = 1, 2, 4 or 8; 0 picks the autotuned width per instance, as
+ * pbe_batch_plan_create does).  Same results. */
+hipError_t pbe_batch_plan_create_ex(pbe_batch_plan* out,
+                                    const pbe_batch_instance* instances, int count,
+                                    int kernel_type, int selection, int n_quad,
+                                    int tile_u);
 hipError_t pbe_batch_plan_destroy(pbe_batch_plan plan);
 /* Length of the packed arrays the plan addresses (max offset + n). */
 int pbe_batch_plan_size(pbe_batch_plan plan);

@@ -28,48 +28,57 @@ static_assert(offsetof(pbe_breakage_params, block_size) == 48, "layout");
 namespace pbe {
 
 // Columns per lane and band-tile row slices of the aggregation kernels, from
-// the autotuned table (tile_table.inc: rows {n_max, kernel_type, tile_u,
-// band_split}, generated on an MI355X by pbe-fixed-pivot-cuda_amd/autotune.py;
-// kernel_type -1 matches any).
-struct TileRule { int n_max, kernel, u, w; };
+// the autotuned table (tile_table.inc: rows {n_max, kernel_type, reach_max,
+// tile_u, band_split}, written by pbe-fixed-pivot-cuda_amd/autotune.py from
+// timings on an MI355X).  The reach of a geometric grid, log_r(1 / (r - 1))
+// bins, is how far below the diagonal a pair (j, k) can still be near (its
+// birth beyond the bracket (k, k+1)); reach_max -1 rows are for grids without
+// the geometric lookup (inv_log_r = 0).  First match wins.
+struct TileRule { int n_max, kernel, reach_max, u, w; };
 static const TileRule kTileTable[] = {
 #include "tile_table.inc"
 };
 
-static const TileRule& tile_rule(int n, int kernel_type)
+double grid_reach(double inv_log_r)
 {
-    static const TileRule fallback{1 << 30, -1, 4, 4};
-    for (const TileRule& r : kTileTable)
-        if (n <= r.n_max && (r.kernel < 0 || r.kernel == kernel_type)) return r;
-    return fallback;
+    if (!(inv_log_r > 0.0) || !std::isfinite(inv_log_r)) return -1.0;
+    const double rm1 = std::expm1(1.0 / inv_log_r);            // r - 1
+    if (!(rm1 > 0.0)) return -1.0;
+    const double reach = -std::log(rm1) * inv_log_r;
+    return reach > 0.0 ? reach : 0.0;
 }
 
-int default_tile_u(int n, int kernel_type) { return tile_rule(n, kernel_type).u; }
+// Tile width of the batched form's pair tiles (batch_tile_table.inc: rows
+// {n_max, kernel_type, reach_max, tile_u}, from the same tuner; the band
+// tiles always run as four row slices there).
+struct BatchTileRule { int n_max, kernel, reach_max, u; };
+static const BatchTileRule kBatchTileTable[] = {
+#include "batch_tile_table.inc"
+};
 
-int default_band_split(int n, int kernel_type) { return tile_rule(n, kernel_type).w; }
-
-// The table's columns per lane, widened on fine geometric grids.  A strip
-// pair (j, k) lies more than T = 64 U bins off the diagonal, and it is far
-// (both births in bins k, k+1) when x_j < (r - 1) x_k, i.e. k - j > log_r
-// (1 / (r - 1)) bins; a strip chunk holding a nearer pair takes the per-pair
-// path for all its pairs, and those chunks sit together in one or two
-// workgroups.  So from n = 6000 up, a grid whose reach exceeds T but not 512
-// bins takes U = 8, which leaves the strips far pairs only; a finer grid
-// takes U = 8 from n = 32768 up (fewer, wider near chunks).  Measured, one
-// aggregation call, U = 8 against U = 4: n = 6000, r = 1.01 (reach 463 bins)
-// 0.067 / 0.107 ms; n = 8192, r = 1.01 0.070 / 0.184 ms; n = 16384, r = 1.01
-// 0.12 / 0.40 ms; n = 49152, r = 1.006 (reach 866) 2.58 / 3.81 ms.  Below
-// n = 6000 the fused launch at U = 4 is faster (n = 4096, r = 1.01: 0.061
-// against 0.138 ms), as is U = 4 for the finer grid at n = 16384 (r = 1.005:
-// 0.53 against 0.57 ms).  General grids keep the table's choice.
-int grid_tile_u(const pbe_aggregation_params& p)
+static bool reach_matches(double reach, int reach_max)
 {
-    const int u = default_tile_u(p.n, p.kernel_type);
-    const double il = p.inv_log_r;
-    if (p.n < 6000 || !(il > 0.0) || !std::isfinite(il) || u == 8) return u;
-    const double reach = -std::log(std::expm1(1.0 / il)) * il;   // log_r (1 / (r - 1))
-    if (reach <= 64.0 * u) return u;
-    return (reach <= 512.0 || p.n >= 32768) ? 8 : u;
+    return reach < 0.0 ? reach_max < 0 : (reach_max >= 0 && reach <= reach_max);
+}
+
+int batch_tile_u(int n, int kernel_type, double inv_log_r)
+{
+    const double reach = grid_reach(inv_log_r);
+    for (const BatchTileRule& r : kBatchTileTable)
+        if (n <= r.n_max && (r.kernel < 0 || r.kernel == kernel_type) &&
+            reach_matches(reach, r.reach_max))
+            return r.u;
+    return 2;
+}
+
+TileChoice tile_choice(int n, int kernel_type, double inv_log_r)
+{
+    const double reach = grid_reach(inv_log_r);
+    for (const TileRule& r : kTileTable)
+        if (n <= r.n_max && (r.kernel < 0 || r.kernel == kernel_type) &&
+            reach_matches(reach, r.reach_max))
+            return TileChoice{r.u, r.w};
+    return TileChoice{4, 4};
 }
 
 }  // namespace pbe
@@ -83,10 +92,8 @@ extern "C" hipError_t pbe_launch_aggregation_rhs(const double* N, const double* 
     if (!N || !x || !rhs) return hipErrorInvalidDevicePointer;
     if (p->kernel_type < 0 || p->kernel_type > 5) return hipErrorInvalidValue;
     if (p->block_size > 1024) return hipErrorInvalidConfiguration;
-    hipError_t e = pbe::launch_aggregation(N, x, rhs, *p,
-                                           pbe::grid_tile_u(*p),
-                                           pbe::default_band_split(p->n, p->kernel_type),
-                                           stream);
+    const pbe::TileChoice t = pbe::tile_choice(p->n, p->kernel_type, p->inv_log_r);
+    hipError_t e = pbe::launch_aggregation(N, x, rhs, *p, t.u, t.w, stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }
@@ -101,10 +108,11 @@ extern "C" hipError_t pbe_launch_aggregation_rhs_ex(const double* N, const doubl
     if (!N || !x || !rhs) return hipErrorInvalidDevicePointer;
     if (p->kernel_type < 0 || p->kernel_type > 5) return hipErrorInvalidValue;
     if (p->block_size > 1024) return hipErrorInvalidConfiguration;
+    const pbe::TileChoice t = pbe::tile_choice(p->n, p->kernel_type, p->inv_log_r);
     int u = opts ? opts->tile_u : 0;
     int w = opts ? opts->band_split : 0;
-    if (u == 0) u = pbe::grid_tile_u(*p);
-    if (w == 0) w = pbe::default_band_split(p->n, p->kernel_type);
+    if (u == 0) u = t.u;
+    if (w == 0) w = t.w;
     if (u != 1 && u != 2 && u != 4 && u != 8) return hipErrorInvalidValue;
     if (w != 1 && w != 2 && w != 4 && w != 8 && w != 16) return hipErrorInvalidValue;
     hipError_t e = pbe::launch_aggregation(N, x, rhs, *p, u, w, stream);
@@ -112,9 +120,15 @@ extern "C" hipError_t pbe_launch_aggregation_rhs_ex(const double* N, const doubl
     return hipGetLastError();
 }
 
-extern "C" int pbe_aggregation_default_tile(int n, int kernel_type)
+extern "C" hipError_t pbe_aggregation_default_options(const pbe_aggregation_params* p,
+                                                      pbe_agg_options* out)
 {
-    return pbe::default_tile_u(n, kernel_type);
+    if (!p || !out || p->n <= 0) return hipErrorInvalidValue;
+    if (p->kernel_type < 0 || p->kernel_type > 5) return hipErrorInvalidValue;
+    const pbe::TileChoice t = pbe::tile_choice(p->n, p->kernel_type, p->inv_log_r);
+    out->tile_u = t.u;
+    out->band_split = t.w;
+    return hipSuccess;
 }
 
 extern "C" hipError_t pbe_launch_breakage_rhs(const double* N, const double* x,
@@ -321,7 +335,15 @@ extern "C" hipError_t pbe_batch_plan_create(pbe_batch_plan* out,
                                             const pbe_batch_instance* instances, int count,
                                             int kernel_type, int selection, int n_quad)
 {
-    return pbe::batch_plan_create(out, instances, count, kernel_type, selection, n_quad);
+    return pbe::batch_plan_create(out, instances, count, kernel_type, selection, n_quad, 0);
+}
+
+extern "C" hipError_t pbe_batch_plan_create_ex(pbe_batch_plan* out,
+                                               const pbe_batch_instance* instances, int count,
+                                               int kernel_type, int selection, int n_quad,
+                                               int tile_u)
+{
+    return pbe::batch_plan_create(out, instances, count, kernel_type, selection, n_quad, tile_u);
 }
 
 extern "C" hipError_t pbe_batch_plan_destroy(pbe_batch_plan plan)

@@ -17,10 +17,13 @@ hipError_t launch_breakage(const double* N, const double* x, const double* t_q,
                            const double* bw_q, double* rhs,
                            const pbe_breakage_params& p, hipStream_t stream);
 
-// Columns per lane picked for an aggregation problem of size n.
-int default_tile_u(int n, int kernel_type);
-// The same, widened to 8 on a fine geometric grid (capi.hip).
-int grid_tile_u(const pbe_aggregation_params& p);
+// Launch shape of the aggregation kernels for a problem (the autotuned
+// table, capi.hip): columns per lane and band-tile row slices.
+struct TileChoice { int u, w; };
+TileChoice tile_choice(int n, int kernel_type, double inv_log_r);
+// Far-pair reach log_r(1 / (r - 1)) in bins of a geometric grid (>= 0), -1
+// for inv_log_r = 0 (no geometric lookup).
+double grid_reach(double inv_log_r);
 
 // Atomic-free, bitwise repeatable forms (det_kernels.hip); ws holds
 // det_*_workspace_doubles(n) doubles.
@@ -40,7 +43,6 @@ hipError_t launch_breakage_deterministic(const double* N, const double* x, const
                                          const double* bw_q, double* rhs,
                                          const pbe_breakage_params& p, double* ws,
                                          hipStream_t stream);
-int default_band_split(int n, int kernel_type);
 
 // One right-hand side R(N) = R_agg(N) + R_brk(N); either part may be absent.
 // With `batch` set, R is the batched source of every instance of the plan and
@@ -56,7 +58,10 @@ struct RhsProblem {
 };
 
 hipError_t batch_plan_create(pbe_batch_plan* out, const pbe_batch_instance* instances,
-                             int count, int kernel_type, int selection, int n_quad);
+                             int count, int kernel_type, int selection, int n_quad,
+                             int tile_u);
+// Columns per lane of one batched instance's pair tiles (capi.hip).
+int batch_tile_u(int n, int kernel_type, double inv_log_r);
 hipError_t batch_plan_destroy(pbe_batch_plan plan);
 int batch_plan_total(pbe_batch_plan plan);
 bool batch_plan_needs_table(pbe_batch_plan plan);

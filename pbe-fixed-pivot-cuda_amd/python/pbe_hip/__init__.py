@@ -128,6 +128,16 @@ def launch_aggregation_rhs_ex(N, x, rhs, params, tile_u=0, band_split=0, stream=
                                                    _stream(stream))
 
 
+def aggregation_default_options(params):
+    """(tile_u, band_split) that options of 0 resolve to for these params
+    (pbe_aggregation_default_options)."""
+    opt = AggOptions(0, 0)
+    hip.check(library().pbe_aggregation_default_options(ctypes.byref(params),
+                                                        ctypes.byref(opt)),
+              "pbe_aggregation_default_options")
+    return opt.tile_u, opt.band_split
+
+
 def launch_aggregation_rhs_deterministic(N, x, rhs, params, workspace, stream=None):
     """rhs += aggregation source, atomic-free and bitwise repeatable
     (pbe_launch_aggregation_rhs_deterministic).  `workspace` is a device array
@@ -284,7 +294,7 @@ class BatchPlan:
     the coefficients (log_x0, inv_log_r, beta0, beta_br, beta_sh, S0, v_ref,
     alpha, v_min).  kernel_type / selection = -1 switch a term off."""
 
-    def __init__(self, instances, kernel_type, selection, n_quad):
+    def __init__(self, instances, kernel_type, selection, n_quad, tile_u=0):
         arr = (BatchInstance * len(instances))()
         for a, d in zip(arr, instances):
             a.n, a.offset = int(d["n"]), int(d["offset"])
@@ -292,10 +302,17 @@ class BatchPlan:
                       "alpha", "v_min"):
                 setattr(a, f, float(d.get(f, 0.0)))
         self.handle = ctypes.c_void_p()
-        hip.check(library().pbe_batch_plan_create(ctypes.byref(self.handle), arr,
-                                                  len(instances), int(kernel_type),
-                                                  int(selection), int(n_quad)),
-                  "pbe_batch_plan_create")
+        if tile_u:
+            hip.check(library().pbe_batch_plan_create_ex(ctypes.byref(self.handle), arr,
+                                                         len(instances), int(kernel_type),
+                                                         int(selection), int(n_quad),
+                                                         int(tile_u)),
+                      "pbe_batch_plan_create_ex")
+        else:
+            hip.check(library().pbe_batch_plan_create(ctypes.byref(self.handle), arr,
+                                                      len(instances), int(kernel_type),
+                                                      int(selection), int(n_quad)),
+                      "pbe_batch_plan_create")
         self.size = library().pbe_batch_plan_size(self.handle)
         self.instances = instances
 

@@ -8,7 +8,7 @@ vp, i, d, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
 def declare(lib):
     sig = {
         "pbe_launch_aggregation_rhs_ex": (i, [vp, vp, vp, vp, vp, vp]),
-        "pbe_aggregation_default_tile": (i, [i, i]),
+        "pbe_aggregation_default_options": (i, [vp, vp]),
         "pbe_aggregation_deterministic_workspace_bytes": (sz, [i]),
         "pbe_breakage_deterministic_workspace_bytes": (sz, [i]),
         "pbe_launch_aggregation_rhs_deterministic": (i, [vp, vp, vp, vp, vp, sz, vp]),
@@ -21,6 +21,7 @@ def declare(lib):
         "pbe_graph_launch": (i, [vp, vp]),
         "pbe_graph_destroy": (i, [vp]),
         "pbe_batch_plan_create": (i, [ctypes.POINTER(vp), vp, i, i, i, i]),
+        "pbe_batch_plan_create_ex": (i, [ctypes.POINTER(vp), vp, i, i, i, i, i]),
         "pbe_batch_plan_destroy": (i, [vp]),
         "pbe_batch_plan_size": (i, [vp]),
         "pbe_launch_batched_rhs": (i, [vp, vp, vp, vp, vp, vp, vp]),

@@ -185,6 +185,71 @@ def test_aggregation_option_errors(pbe):
                                                  ctypes.byref(opt), None) == INVALID_VALUE
 
 
+def _table_rows(name, width):
+    path = os.path.join(REPO, "pbe-fixed-pivot-cuda_amd", "csrc", name)
+    rows = [tuple(int(v) for v in m.group(1).split(","))
+            for m in re.finditer(r"^\{([-0-9, ]+)\},$", open(path).read(), flags=re.M)]
+    assert rows and all(len(r) == width for r in rows)
+    return rows
+
+
+@pytest.mark.parametrize("name,width", [("tile_table.inc", 5), ("batch_tile_table.inc", 4)])
+def test_tile_tables_cover_every_case(name, width):
+    """Every kernel and every grid (without the geometric lookup, and every
+    reach) finds a row at every n: per (kernel, reach class) the rows' n_max
+    increase and the last is open, and each kernel has a row of reach -1 and
+    one of open reach.  Shapes are ones the launches take."""
+    rows = _table_rows(name, width)
+    groups = {}
+    for r in rows:
+        assert r[1] in range(-1, 6) and r[3] in (1, 2, 4, 8)
+        if width == 5:
+            assert r[4] in (1, 2, 4, 8, 16)
+        groups.setdefault((r[1], r[2]), []).append(r[0])
+    for key, nmax in groups.items():
+        assert nmax == sorted(nmax) and len(set(nmax)) == len(nmax), key
+        assert nmax[-1] == 1 << 30, key
+    for k in range(6):
+        reaches = {c for (kk, c) in groups if kk in (k, -1)}
+        assert -1 in reaches and (1 << 30) in reaches, (name, k)
+
+
+def test_default_options_follow_the_table(pbe):
+    """pbe_aggregation_default_options resolves (n, kernel, reach) through
+    the first matching row of tile_table.inc; bad arguments are refused."""
+    import math
+    rows = _table_rows("tile_table.inc", 5)
+    for n, r, k in ((300, 1.02, 5), (5000, 1.01, 3), (70000, 1.002, 0), (2048, None, 2),
+                    (1000, 2.0, 1)):
+        il = 0.0 if r is None else 1.0 / math.log(r)
+        reach = -1.0 if r is None else max(0.0, math.log(1.0 / (r - 1.0)) / math.log(r))
+        want = next((u, w) for (nm, kk, rm, u, w) in rows
+                    if n <= nm and kk in (k, -1)
+                    and (rm < 0 if reach < 0 else (rm >= 0 and reach <= rm)))
+        got = pbe.aggregation_default_options(pbe.AggregationParams(n=n, kernel_type=k,
+                                                                    inv_log_r=il))
+        assert got == want, (n, r, k)
+    lib = pbe.library()
+    opt = pbe.AggOptions(0, 0)
+    for A in (pbe.AggregationParams(n=0), pbe.AggregationParams(n=10, kernel_type=6)):
+        assert lib.pbe_aggregation_default_options(ctypes.byref(A),
+                                                   ctypes.byref(opt)) == INVALID_VALUE
+    assert lib.pbe_aggregation_default_options(None, ctypes.byref(opt)) == INVALID_VALUE
+
+
+def test_batch_plan_tile_width_refused():
+    """pbe_batch_plan_create_ex refuses a tile width other than 0, 1, 2, 4, 8
+    before any device work."""
+    import pbe_hip as pbe
+    inst = (pbe.BatchInstance * 1)()
+    inst[0].n = 64
+    h = ctypes.c_void_p()
+    for u in (3, 16, -1):
+        assert pbe.library().pbe_batch_plan_create_ex(ctypes.byref(h), inst, 1, 5, -1, 0,
+                                                      u) == INVALID_VALUE
+        assert not h.value
+
+
 def test_torch_solve_rejects_host_tensors():
     """PBESolve refuses a host tensor before any device work (its pointer
     would otherwise reach the kernels)."""

@@ -96,3 +96,44 @@ def test_batched_rk4_and_graph(pbe):
         a, b = Ng.download(), Ne.download()
         assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max(), rep
     g.destroy()
+
+
+@pytest.mark.parametrize("tile_u", [1, 2, 4, 8])
+@pytest.mark.parametrize("kernel", [5, 2])
+def test_batched_every_tile_width_vs_oracle(pbe, oracle_mod, tile_u, kernel):
+    """Every forced tile width of the batched aggregation (64 tile_u wide
+    pair tiles, one launch per width) against the oracle per instance, on
+    geometric instances and on instances without the geometric lookup
+    (inv_log_r = 0, jittered grids), n from 2 to 1100."""
+    from pbe_hip import hip, problems as P
+    rng = np.random.default_rng(100 * tile_u + kernel)
+    inst, xs, Ns = [], [], []
+    off = 0
+    for i, n in enumerate([2, 3, 63, 64, 65, 129, 255, 256, 257, 511, 700, 1100]):
+        general = i % 3 == 2
+        x = (P.jittered_grid(n, 1e-18, 1.03, 0.3, seed=i) if general
+             else P.geometric_grid(n, 1e-18, 1.03 if i % 2 else 1.01))
+        N = P.lognormal_ic(x, x[n // 3], 1.0, 1e12, noise=0.05, seed=i)
+        b0 = 1e-17 / (x[n // 4] ** 2 if kernel == 2 else 1.0)
+        inst.append(dict(n=n, offset=off, beta0=b0, beta_br=6.73e-18,
+                         beta_sh=4.0 / 3.0 * float(rng.uniform(10, 500)),
+                         log_x0=0.0 if general else float(np.log(x[0])),
+                         inv_log_r=0.0 if general else float(1.0 / np.log(x[1] / x[0]))))
+        xs.append(x)
+        Ns.append(N)
+        off += n
+    x, N = np.concatenate(xs), np.concatenate(Ns)
+    plan = pbe.BatchPlan(inst, kernel_type=kernel, selection=-1, n_quad=0, tile_u=tile_u)
+    dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
+    rhs = hip.DeviceArray(len(N))
+    rhs.zero()
+    hip.check(plan.launch(dN, dx, None, None, rhs))
+    hip.synchronize()
+    got = rhs.download()
+    plan.destroy()
+    for s in inst:
+        lo, hi = s["offset"], s["offset"] + s["n"]
+        agg = dict(kernel=kernel, beta0=s["beta0"], beta_br=s["beta_br"], beta_sh=s["beta_sh"],
+                   log_x0=s["log_x0"], inv_log_r=s["inv_log_r"])
+        ra, Aa, _ = oracle_mod.aggregation_rhs(N[lo:hi], x[lo:hi], **agg)
+        assert_parity(got[lo:hi], ra, Aa, what=f"tile_u={tile_u} n={s['n']}")
