@@ -8,6 +8,9 @@
  * bracket rule (log-ratio formula when inv_log_r != 0, binary search
  * otherwise).  Work split: rows j of the upper triangle dealt out dynamically
  * to threads, each thread accumulates into a private rhs, then a reduction.
+ * Built at -O3 with the collision kernel inlined per kernel type and the
+ * row loops cloned for AVX-512 / AVX2 / baseline, picked at load time for
+ * the host that runs it.
  */
 #include "pbe_oracle.h"
 
@@ -18,11 +21,80 @@
 #include <omp.h>
 #endif
 
-static inline double beta_of(int kernel, double xj, double xk, double b0,
-                             double b_br, double b_sh)
+/* beta(x_j, x_k) of src/aggregation_kernels.cuh:45-83 with the bins' cube
+ * roots taken from a per-call table (cbrt is a pure function of x, so the
+ * values are the reference's).  `kernel` is a compile-time constant at every
+ * call site below, so the switch folds away and the expression inlines. */
+static inline __attribute__((always_inline)) double port_beta(
+    const int kernel, double xj, double xk, double cj, double ck, double b0,
+    double b_br, double b_sh)
 {
-    return oracle_beta(kernel, xj, xk, b0, b_br, b_sh);
+    switch (kernel) {
+    case 0: return b0;
+    case 1: return b0 * (xj + xk);
+    case 2: return b0 * xj * xk;
+    case 3: return b_br * (cj / ck + ck / cj + 2.0);
+    case 4: { const double s = cj + ck; return b_sh * s * s * s; }
+    case 5: {
+        const double s = cj + ck;
+        return b_br * (cj / ck + ck / cj + 2.0) + b_sh * s * s * s;
+    }
+    default: return 0.0;
+    }
 }
+
+/* One row j of the upper triangle into the private rhs r: rates of the row
+ * first (a loop the compiler vectorises), then deaths, then births with the
+ * reference's bracket rule (log-ratio formula when inv_log_r != 0, binary
+ * search otherwise; src/aggregation_kernels.cuh:118-134,186-231).  The
+ * clones pick AVX-512, AVX2 or baseline code for the host that runs it. */
+#define PORT_AGG_ROW(K)                                                                   \
+__attribute__((target_clones("avx512f", "avx2", "default")))                              \
+static void port_agg_row_##K(int n, int j, const double* N, const double* x,               \
+                             const double* cb, double b0, double b_br, double b_sh,        \
+                             double log_x0, double inv_log_r, double* rate, double* r)    \
+{                                                                                          \
+    const double xj = x[j], Nj = N[j], cj = cb[j];                                         \
+    const double xlast = x[n - 1], xfirst = x[0];                                          \
+    for (int k = j; k < n; ++k)                                                            \
+        rate[k] = port_beta(K, xj, x[k], cj, cb[k], b0, b_br, b_sh) * Nj * N[k];           \
+    rate[j] *= 0.5;                                                                        \
+    double dj = 2.0 * rate[j];                                                             \
+    for (int k = j + 1; k < n; ++k) {                                                      \
+        dj += rate[k];                                                                     \
+        r[k] -= rate[k];                                                                   \
+    }                                                                                      \
+    r[j] -= dj;                                                                            \
+    for (int k = j; k < n; ++k) {                                                          \
+        const double v = xj + x[k], c = rate[k];                                           \
+        if (v >= xlast) { r[n - 1] += c; continue; }                                       \
+        if (v <= xfirst) { r[0] += c; continue; }                                          \
+        int hi;                                                                            \
+        if (inv_log_r != 0.0) {                                                            \
+            hi = (int)floor((log(v) - log_x0) * inv_log_r) + 1;                            \
+            if (hi < 1) hi = 1;                                                            \
+            if (hi > n - 1) hi = n - 1;                                                    \
+        } else {                                                                           \
+            int lo = 0;                                                                    \
+            hi = n - 1;                                                                    \
+            while (hi - lo > 1) {                                                          \
+                const int mid = (lo + hi) / 2;                                             \
+                if (x[mid] <= v) lo = mid; else hi = mid;                                  \
+            }                                                                              \
+        }                                                                                  \
+        const int lo = hi - 1;                                                             \
+        const double w = (v - x[lo]) / (x[hi] - x[lo]);                                    \
+        r[lo] += (1.0 - w) * c;                                                            \
+        r[hi] += w * c;                                                                    \
+    }                                                                                      \
+}
+
+PORT_AGG_ROW(0)
+PORT_AGG_ROW(1)
+PORT_AGG_ROW(2)
+PORT_AGG_ROW(3)
+PORT_AGG_ROW(4)
+PORT_AGG_ROW(5)
 
 int port_aggregation_rhs(int n, const double* N, const double* x, int kernel,
                          double beta0, double beta_br, double beta_sh,
@@ -33,9 +105,12 @@ int port_aggregation_rhs(int n, const double* N, const double* x, int kernel,
     if (!N || !x || !rhs) return ORACLE_ERR_INVALID_POINTER;
     if (kernel < 0 || kernel > 5) return ORACLE_ERR_INVALID_VALUE;
     if (threads < 1) threads = 1;
-    double* priv = (double*)calloc((size_t)n * (size_t)threads, sizeof(double));
+    /* per thread: a private rhs and a row of rates; shared: cube roots */
+    double* priv = (double*)calloc((size_t)2 * (size_t)n * (size_t)threads + (size_t)n,
+                                   sizeof(double));
     if (!priv) return ORACLE_ERR_NO_MEMORY;
-    const double xlast = x[n - 1], xfirst = x[0];
+    double* cb = priv + (size_t)2 * (size_t)n * (size_t)threads;
+    for (int i = 0; i < n; ++i) cb[i] = cbrt(x[i]);
 
 #pragma omp parallel num_threads(threads)
     {
@@ -43,41 +118,22 @@ int port_aggregation_rhs(int n, const double* N, const double* x, int kernel,
 #ifdef _OPENMP
         tid = omp_get_thread_num();
 #endif
-        double* r = priv + (size_t)tid * (size_t)n;
+        double* r = priv + (size_t)2 * (size_t)tid * (size_t)n;
+        double* rate = r + n;
 #pragma omp for schedule(dynamic, 8)
         for (int j = 0; j < n; ++j) {
-            const double xj = x[j], Nj = N[j];
-            for (int k = j; k < n; ++k) {
-                const double xk = x[k];
-                double rate = beta_of(kernel, xj, xk, beta0, beta_br, beta_sh) * Nj * N[k];
-                if (j == k) { rate *= 0.5; r[j] -= 2.0 * rate; }
-                else        { r[j] -= rate; r[k] -= rate; }
-                const double v = xj + xk;
-                if (v >= xlast)       r[n - 1] += rate;
-                else if (v <= xfirst) r[0] += rate;
-                else {
-                    int hi;
-                    if (inv_log_r != 0.0) {
-                        hi = (int)floor((log(v) - log_x0) * inv_log_r) + 1;
-                        if (hi < 1) hi = 1;
-                        if (hi > n - 1) hi = n - 1;
-                    } else {
-                        int lo = 0; hi = n - 1;
-                        while (hi - lo > 1) {
-                            int mid = (lo + hi) / 2;
-                            if (x[mid] <= v) lo = mid; else hi = mid;
-                        }
-                    }
-                    const int lo = hi - 1;
-                    const double w = (v - x[lo]) / (x[hi] - x[lo]);
-                    r[lo] += (1.0 - w) * rate;
-                    r[hi] += w * rate;
-                }
+            switch (kernel) {
+            case 0: port_agg_row_0(n, j, N, x, cb, beta0, beta_br, beta_sh, log_x0, inv_log_r, rate, r); break;
+            case 1: port_agg_row_1(n, j, N, x, cb, beta0, beta_br, beta_sh, log_x0, inv_log_r, rate, r); break;
+            case 2: port_agg_row_2(n, j, N, x, cb, beta0, beta_br, beta_sh, log_x0, inv_log_r, rate, r); break;
+            case 3: port_agg_row_3(n, j, N, x, cb, beta0, beta_br, beta_sh, log_x0, inv_log_r, rate, r); break;
+            case 4: port_agg_row_4(n, j, N, x, cb, beta0, beta_br, beta_sh, log_x0, inv_log_r, rate, r); break;
+            default: port_agg_row_5(n, j, N, x, cb, beta0, beta_br, beta_sh, log_x0, inv_log_r, rate, r); break;
             }
         }
     }
     for (int t = 0; t < threads; ++t)
-        for (int i = 0; i < n; ++i) rhs[i] += priv[(size_t)t * (size_t)n + i];
+        for (int i = 0; i < n; ++i) rhs[i] += priv[(size_t)2 * (size_t)t * (size_t)n + i];
     free(priv);
     return 0;
 }
