@@ -371,10 +371,26 @@ def main():
 
     flops = P.PAIR_FLOPS[int(agg["kernel"])] * pairs
     achieved = flops / (agg_ms * 1e-3) / 1e12
+    # the launches the library's tuned shape makes for this call
+    tu, tw, tf = pbe_hip.aggregation_default_options(ap)
+    kname = pbe_hip.AggregationKernel(int(agg["kernel"])).name
+    if tu == 4 and tf == 2 and -(-n // 256) <= 64:
+        agg_label = f"aggregation call: agg_fused_kernel<{kname}> (band split {tw})"
+    elif tu == 4:
+        agg_label = (f"aggregation call: agg_sweep_kernel<{kname}> (band split {tw}) + "
+                     f"agg_strip_kernel<{kname}, 4>")
+    else:
+        agg_label = (f"aggregation call: {64 * tu}-column tiles and strips <{kname}> "
+                     f"(band split {tw})")
+    # HBM bytes per call from the committed counter runs of the same launches
+    # (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-kernel calibrated); null when
+    # they were taken on other launches
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        if tj.get("launch") == agg_label:
+            traffic = tj.get("hbm_bytes_per_launch")
 
     out = {
         "metric": METRIC,
@@ -407,7 +423,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": traffic,
-            "kernel": "aggregation call: agg_sweep_kernel + agg_strip_kernel <BrownianShear>",
+            "kernel": agg_label,
             "flop_per_launch": flops,
             "avg_launch_ms": agg_ms,
             "median_launch_ms": agg_median,

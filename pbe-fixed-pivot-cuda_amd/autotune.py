@@ -5,11 +5,12 @@ library compiles in.
         [--out-dir DIR] [--summary FILE] [--quick]
 
 Single-instance launches (csrc/tile_table.inc, rows {n_max, kernel_type,
-reach_max, U, S}): for every collision kernel, grid-reach class and size
+reach_max, U, S, F}): for every collision kernel, grid-reach class and size
 bucket, every block width 64*U (U = 1, 2, 4, 8 columns per lane) and band
 split S = 1, 2, 4, 8, 16 (row slices per diagonal-band tile; offset segments
-per block for the 256-wide band sweeps) is timed, and the fastest (U, S) is
-written.
+per block for the 256-wide band sweeps) is timed, at U = 4 both as two
+launches (F = 1) and as the one fused launch (F = 2), and the fastest
+(U, S, F) is written.
 
 Batched launches (csrc/batch_tile_table.inc, rows {n_max, kernel_type,
 reach_max, U}): for every kernel, reach class and instance-size bucket, a
@@ -130,7 +131,7 @@ def class_sizes(sizes, r, spread=1.0):
 
 
 def tune_single(pbe, hip, timer, sizes, kernels, log):
-    """{(kernel, reach_max): [(n, {(U, S): t_ms})]}"""
+    """{(kernel, reach_max): [(n, {(U, S, F): t_ms})]}"""
     best = {}
     for ci, (reach_max, r) in enumerate(CLASSES):
         general = reach_max < 0
@@ -147,16 +148,17 @@ def tune_single(pbe, hip, timer, sizes, kernels, log):
                     agg["log_x0"], agg["inv_log_r"] = math.log(x[0]), 1.0 / math.log(x[1] / x[0])
                 p = pbe.agg_params(n, agg)
 
-                def run(u, w):
+                def run(u, w, f):
                     def launch(s):
                         hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=u,
-                                                                band_split=w, stream=s),
-                                  f"n {n} kernel {k} tile {u} split {w}")
+                                                                band_split=w, form=f, stream=s),
+                                  f"n {n} kernel {k} tile {u} split {w} form {f}")
                     return timer(launch)
-                t = {(u, w): run(u, w) for u in TILES for w in SPLITS}
-                bu, bw = min(t, key=t.get)
+                t = {(u, w, f): run(u, w, f) for u in TILES for w in SPLITS
+                     for f in ((1, 2) if u == 4 else (1,))}
+                bu, bw, bf = min(t, key=t.get)
                 best.setdefault((k, reach_max), []).append((n, t))
-                line.append(f"k{k}:U{bu}S{bw} {t[(bu, bw)] * 1e3:.1f}us")
+                line.append(f"k{k}:U{bu}S{bw}F{bf} {t[(bu, bw, bf)] * 1e3:.1f}us")
             log(f"single reach<={reach_max if reach_max >= 0 else 'general'} r={r} n={n}: "
                 + "  ".join(line))
     return best
@@ -253,7 +255,7 @@ def summary_lines(best, batch):
         for i, (n, shape, t) in enumerate(choices(ent)):
             pairs = ent[[e[0] for e in ent].index(n)][2] if batch else None
             val = (f"U{shape[0]} {pairs / (t * 1e-3):.2e}/s" if batch
-                   else f"U{shape[0]} S{shape[1]} {t * 1e3:.1f}")
+                   else f"U{shape[0]} S{shape[1]} F{shape[2]} {t * 1e3:.1f}")
             cells[(c, n, k)] = val
     head = "| reach (bins) | n" + (" ~" if batch else "") + " | " + " | ".join(
         f"kernel {k}" for k in ks) + " |"
@@ -296,12 +298,13 @@ def main():
         best = tune_single(pbe, hip, timer, sizes, kernels, log)
         write_table(os.path.join(args.out_dir, "tile_table.inc"),
                     ["{n_max, kernel_type, reach_max (-1: grids without the geometric",
-                     "lookup), columns per lane, band-tile row slices}; first match wins.",
-                     "Generated by pbe-fixed-pivot-cuda_amd/autotune.py on an MI355X",
-                     "(gfx950); do not edit."],
+                     "lookup), columns per lane, band-tile row slices, launch form}; first",
+                     "match wins.  Generated by pbe-fixed-pivot-cuda_amd/autotune.py on an",
+                     "MI355X (gfx950); do not edit."],
                     merged_rows(best))
-        summary += ["Single launches: chosen U (columns per lane), S (band split) and "
-                    "the time of one aggregation call in us.", ""]
+        summary += ["Single launches: chosen U (columns per lane), S (band split), F "
+                    "(launch form: 1 two launches, 2 fused) and the time of one "
+                    "aggregation call in us.", ""]
         summary += summary_lines(best, batch=False)
     if args.part in ("batch", "both"):
         bbest = tune_batch(pbe, hip, timer, bsizes, kernels, log)

@@ -197,13 +197,17 @@ void agg_strip_kernel(AggArgs a, StripPlan plan)
     }
 }
 
-// Mid-size problems: the band sweep items and strip shares of four waves in
-// one launch, side by side (two 256-thread workgroups per CU), so that the
-// two latency chains run together instead of one after the other.  The sweep
-// items come first in the grid; the strip shares are the rest, all resident
-// at once.
+// Mid-size problems (form 2): the band sweep items and strip shares of four
+// waves in one launch, side by side (two 256-thread workgroups per CU), so
+// that the two latency chains run together instead of one after the other
+// (the sweep's births are bound by LDS reads, the strip's by FP64 issue).
+// The sweep items come first in the grid; the strip shares are the rest, all
+// resident at once.  Whether this beats two launches depends on n and on the
+// grid's near-pair reach (measured: n = 16384, r = 1.02, Brownian+shear
+// 56.0 us against 62.4; at r = 1.01 114 against 91), so the tuned table
+// picks it.
 constexpr int kFusedWaves = 4;
-constexpr int kFusedMaxBlocks = 32;   // 256-column blocks (n <= 8192)
+constexpr int kFusedMaxBlocks = 64;   // 256-column blocks (n <= 16384)
 
 template <int KT>
 union FusedSmem {
@@ -315,13 +319,13 @@ static StripPlan make_strip_plan(int n, int U, bool band, double inv_log_r)
 }
 
 template <int KT, int U>
-static hipError_t launch_agg_t(AggArgs args, int split, hipStream_t stream)
+static hipError_t launch_agg_t(AggArgs args, int split, bool fused, hipStream_t stream)
 {
     constexpr int T = AggShape<U>::T;
     args.nb = (args.n + T - 1) / T;
     const long long band_items = (long long)split * band_tiles(args.nb);
     const StripPlan sep = make_strip_plan(args.n, U, false, args.inv_log_r);
-    if (T == kBandT && args.nb <= kFusedMaxBlocks && sep.total > 0) {
+    if (fused && T == kBandT && args.nb <= kFusedMaxBlocks && sep.total > 0) {
         // mid sizes: sweep items on at most half of the resident slots of
         // the fused kernel, strip shares on the rest (empty shares return)
         const long long Gf = fused_resident_groups<KT>();
@@ -360,22 +364,23 @@ static hipError_t launch_agg_t(AggArgs args, int split, hipStream_t stream)
 }
 
 template <int U>
-static hipError_t launch_agg_u(const AggArgs& args, int kernel, int split, hipStream_t stream)
+static hipError_t launch_agg_u(const AggArgs& args, int kernel, int split, bool fused,
+                               hipStream_t stream)
 {
     switch (kernel) {
-    case 0: return launch_agg_t<0, U>(args, split, stream);
-    case 1: return launch_agg_t<1, U>(args, split, stream);
-    case 2: return launch_agg_t<2, U>(args, split, stream);
-    case 3: return launch_agg_t<3, U>(args, split, stream);
-    case 4: return launch_agg_t<4, U>(args, split, stream);
-    case 5: return launch_agg_t<5, U>(args, split, stream);
+    case 0: return launch_agg_t<0, U>(args, split, fused, stream);
+    case 1: return launch_agg_t<1, U>(args, split, fused, stream);
+    case 2: return launch_agg_t<2, U>(args, split, fused, stream);
+    case 3: return launch_agg_t<3, U>(args, split, fused, stream);
+    case 4: return launch_agg_t<4, U>(args, split, fused, stream);
+    case 5: return launch_agg_t<5, U>(args, split, fused, stream);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_aggregation(const double* N, const double* x, double* rhs,
                               const pbe_aggregation_params& p, int tile_u, int band_split,
-                              hipStream_t stream)
+                              int form, hipStream_t stream)
 {
     AggArgs args;
     args.N = N;
@@ -388,11 +393,12 @@ hipError_t launch_aggregation(const double* N, const double* x, double* rhs,
     args.inv_log_r = p.inv_log_r;
     const int S = (band_split == 1 || band_split == 2 || band_split == 8 || band_split == 16)
         ? band_split : 4;
+    const bool fused = form == 2;
     switch (tile_u) {
-    case 1: return launch_agg_u<1>(args, p.kernel_type, S, stream);
-    case 2: return launch_agg_u<2>(args, p.kernel_type, S, stream);
-    case 8: return launch_agg_u<8>(args, p.kernel_type, S, stream);
-    default: return launch_agg_u<4>(args, p.kernel_type, S, stream);
+    case 1: return launch_agg_u<1>(args, p.kernel_type, S, fused, stream);
+    case 2: return launch_agg_u<2>(args, p.kernel_type, S, fused, stream);
+    case 8: return launch_agg_u<8>(args, p.kernel_type, S, fused, stream);
+    default: return launch_agg_u<4>(args, p.kernel_type, S, fused, stream);
     }
 }
 

@@ -29,12 +29,12 @@ namespace pbe {
 
 // Columns per lane and band-tile row slices of the aggregation kernels, from
 // the autotuned table (tile_table.inc: rows {n_max, kernel_type, reach_max,
-// tile_u, band_split}, written by pbe-fixed-pivot-cuda_amd/autotune.py from
+// tile_u, band_split, form}, written by pbe-fixed-pivot-cuda_amd/autotune.py from
 // timings on an MI355X).  The reach of a geometric grid, log_r(1 / (r - 1))
 // bins, is how far below the diagonal a pair (j, k) can still be near (its
 // birth beyond the bracket (k, k+1)); reach_max -1 rows are for grids without
 // the geometric lookup (inv_log_r = 0).  First match wins.
-struct TileRule { int n_max, kernel, reach_max, u, w; };
+struct TileRule { int n_max, kernel, reach_max, u, w, f; };
 static const TileRule kTileTable[] = {
 #include "tile_table.inc"
 };
@@ -77,8 +77,8 @@ TileChoice tile_choice(int n, int kernel_type, double inv_log_r)
     for (const TileRule& r : kTileTable)
         if (n <= r.n_max && (r.kernel < 0 || r.kernel == kernel_type) &&
             reach_matches(reach, r.reach_max))
-            return TileChoice{r.u, r.w};
-    return TileChoice{4, 4};
+            return TileChoice{r.u, r.w, r.f};
+    return TileChoice{4, 4, 1};
 }
 
 }  // namespace pbe
@@ -93,7 +93,7 @@ extern "C" hipError_t pbe_launch_aggregation_rhs(const double* N, const double* 
     if (p->kernel_type < 0 || p->kernel_type > 5) return hipErrorInvalidValue;
     if (p->block_size > 1024) return hipErrorInvalidConfiguration;
     const pbe::TileChoice t = pbe::tile_choice(p->n, p->kernel_type, p->inv_log_r);
-    hipError_t e = pbe::launch_aggregation(N, x, rhs, *p, t.u, t.w, stream);
+    hipError_t e = pbe::launch_aggregation(N, x, rhs, *p, t.u, t.w, t.f, stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }
@@ -111,11 +111,14 @@ extern "C" hipError_t pbe_launch_aggregation_rhs_ex(const double* N, const doubl
     const pbe::TileChoice t = pbe::tile_choice(p->n, p->kernel_type, p->inv_log_r);
     int u = opts ? opts->tile_u : 0;
     int w = opts ? opts->band_split : 0;
+    int f = opts ? opts->form : 0;
     if (u == 0) u = t.u;
     if (w == 0) w = t.w;
+    if (f == 0) f = t.f;
     if (u != 1 && u != 2 && u != 4 && u != 8) return hipErrorInvalidValue;
     if (w != 1 && w != 2 && w != 4 && w != 8 && w != 16) return hipErrorInvalidValue;
-    hipError_t e = pbe::launch_aggregation(N, x, rhs, *p, u, w, stream);
+    if (f != 1 && f != 2) return hipErrorInvalidValue;
+    hipError_t e = pbe::launch_aggregation(N, x, rhs, *p, u, w, f, stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }
@@ -128,6 +131,7 @@ extern "C" hipError_t pbe_aggregation_default_options(const pbe_aggregation_para
     const pbe::TileChoice t = pbe::tile_choice(p->n, p->kernel_type, p->inv_log_r);
     out->tile_u = t.u;
     out->band_split = t.w;
+    out->form = t.f;
     return hipSuccess;
 }
 

@@ -72,7 +72,7 @@ hipError_t enqueue_rhs(const RhsProblem& p, const double* Nin, double* rhs, hipS
     if (p.batch) return enqueue_batched_rhs(p.batch, Nin, p.x, p.t_q, p.bw_q, rhs, s);
     if (p.agg) {
         const TileChoice t = tile_choice(p.agg->n, p.agg->kernel_type, p.agg->inv_log_r);
-        e = launch_aggregation(Nin, p.x, rhs, *p.agg, t.u, t.w, s);
+        e = launch_aggregation(Nin, p.x, rhs, *p.agg, t.u, t.w, t.f, s);
         if (e != hipSuccess) return e;
     }
     if (p.brk) {

@@ -8,9 +8,11 @@
 namespace pbe {
 
 // Enqueue the aggregation tiles (U = columns per lane: 2, 4 or 8).
+// form 2: band and strips in one launch where the shape allows it (tile_u
+// = 4, at most 64 column blocks); 1: two launches.
 hipError_t launch_aggregation(const double* N, const double* x, double* rhs,
                               const pbe_aggregation_params& p, int tile_u, int band_split,
-                              hipStream_t stream);
+                              int form, hipStream_t stream);
 
 // Enqueue the fused breakage death + birth kernel.
 hipError_t launch_breakage(const double* N, const double* x, const double* t_q,
@@ -18,8 +20,8 @@ hipError_t launch_breakage(const double* N, const double* x, const double* t_q,
                            const pbe_breakage_params& p, hipStream_t stream);
 
 // Launch shape of the aggregation kernels for a problem (the autotuned
-// table, capi.hip): columns per lane and band-tile row slices.
-struct TileChoice { int u, w; };
+// table, capi.hip): columns per lane, band-tile row slices and launch form.
+struct TileChoice { int u, w, f; };
 TileChoice tile_choice(int n, int kernel_type, double inv_log_r);
 // Far-pair reach log_r(1 / (r - 1)) in bins of a geometric grid (>= 0), -1
 // for inv_log_r = 0 (no geometric lookup).

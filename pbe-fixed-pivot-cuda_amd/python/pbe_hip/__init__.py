@@ -52,9 +52,11 @@ class AggregationParams(ctypes.Structure):
 
 
 class AggOptions(ctypes.Structure):
-    """pbe_agg_options: block width (columns per lane) and row slices per
-    diagonal-band tile; 0 selects the autotuned value."""
-    _fields_ = [("tile_u", ctypes.c_int), ("band_split", ctypes.c_int)]
+    """pbe_agg_options: block width (columns per lane), row slices per
+    diagonal-band tile and launch form (1: band and strips apart, 2: one
+    launch); 0 selects the autotuned value."""
+    _fields_ = [("tile_u", ctypes.c_int), ("band_split", ctypes.c_int),
+                ("form", ctypes.c_int)]
 
 
 class BreakageParams(ctypes.Structure):
@@ -120,22 +122,22 @@ def launch_aggregation_rhs(N, x, rhs, params, stream=None):
                                                 ctypes.byref(params), _stream(stream))
 
 
-def launch_aggregation_rhs_ex(N, x, rhs, params, tile_u=0, band_split=0, stream=None):
+def launch_aggregation_rhs_ex(N, x, rhs, params, tile_u=0, band_split=0, form=0, stream=None):
     """pbe_launch_aggregation_rhs_ex with explicit launch options."""
-    opt = AggOptions(int(tile_u), int(band_split))
+    opt = AggOptions(int(tile_u), int(band_split), int(form))
     return library().pbe_launch_aggregation_rhs_ex(_ptr(N), _ptr(x), _ptr(rhs),
                                                    ctypes.byref(params), ctypes.byref(opt),
                                                    _stream(stream))
 
 
 def aggregation_default_options(params):
-    """(tile_u, band_split) that options of 0 resolve to for these params
-    (pbe_aggregation_default_options)."""
-    opt = AggOptions(0, 0)
+    """(tile_u, band_split, form) that options of 0 resolve to for these
+    params (pbe_aggregation_default_options)."""
+    opt = AggOptions(0, 0, 0)
     hip.check(library().pbe_aggregation_default_options(ctypes.byref(params),
                                                         ctypes.byref(opt)),
               "pbe_aggregation_default_options")
-    return opt.tile_u, opt.band_split
+    return opt.tile_u, opt.band_split, opt.form
 
 
 def launch_aggregation_rhs_deterministic(N, x, rhs, params, workspace, stream=None):

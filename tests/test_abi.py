@@ -175,12 +175,13 @@ def test_deterministic_argument_errors(pbe):
 
 
 def test_aggregation_option_errors(pbe):
-    """tile_u outside {0,1,2,4,8} or band_split outside {0,1,2,4,8,16} are
-    refused."""
+    """tile_u outside {0,1,2,4,8}, band_split outside {0,1,2,4,8,16} or form
+    outside {0,1,2} are refused."""
     lib = pbe.library()
     A = pbe.AggregationParams(n=100, kernel_type=5)
-    for u, w in ((3, 0), (16, 0), (4, 3), (4, 32), (4, -1)):
-        opt = pbe.AggOptions(u, w)
+    for u, w, f in ((3, 0, 0), (16, 0, 0), (4, 3, 0), (4, 32, 0), (4, -1, 0), (4, 4, 3),
+                    (4, 4, -1)):
+        opt = pbe.AggOptions(u, w, f)
         assert lib.pbe_launch_aggregation_rhs_ex(FAKE, FAKE, FAKE, ctypes.byref(A),
                                                  ctypes.byref(opt), None) == INVALID_VALUE
 
@@ -193,7 +194,7 @@ def _table_rows(name, width):
     return rows
 
 
-@pytest.mark.parametrize("name,width", [("tile_table.inc", 5), ("batch_tile_table.inc", 4)])
+@pytest.mark.parametrize("name,width", [("tile_table.inc", 6), ("batch_tile_table.inc", 4)])
 def test_tile_tables_cover_every_case(name, width):
     """Every kernel and every grid (without the geometric lookup, and every
     reach) finds a row at every n: per (kernel, reach class) the rows' n_max
@@ -203,8 +204,8 @@ def test_tile_tables_cover_every_case(name, width):
     groups = {}
     for r in rows:
         assert r[1] in range(-1, 6) and r[3] in (1, 2, 4, 8)
-        if width == 5:
-            assert r[4] in (1, 2, 4, 8, 16)
+        if width == 6:
+            assert r[4] in (1, 2, 4, 8, 16) and r[5] in (1, 2)
         groups.setdefault((r[1], r[2]), []).append(r[0])
     for key, nmax in groups.items():
         assert nmax == sorted(nmax) and len(set(nmax)) == len(nmax), key
@@ -218,19 +219,19 @@ def test_default_options_follow_the_table(pbe):
     """pbe_aggregation_default_options resolves (n, kernel, reach) through
     the first matching row of tile_table.inc; bad arguments are refused."""
     import math
-    rows = _table_rows("tile_table.inc", 5)
+    rows = _table_rows("tile_table.inc", 6)
     for n, r, k in ((300, 1.02, 5), (5000, 1.01, 3), (70000, 1.002, 0), (2048, None, 2),
                     (1000, 2.0, 1)):
         il = 0.0 if r is None else 1.0 / math.log(r)
         reach = -1.0 if r is None else max(0.0, math.log(1.0 / (r - 1.0)) / math.log(r))
-        want = next((u, w) for (nm, kk, rm, u, w) in rows
+        want = next((u, w, f) for (nm, kk, rm, u, w, f) in rows
                     if n <= nm and kk in (k, -1)
                     and (rm < 0 if reach < 0 else (rm >= 0 and reach <= rm)))
         got = pbe.aggregation_default_options(pbe.AggregationParams(n=n, kernel_type=k,
                                                                     inv_log_r=il))
         assert got == want, (n, r, k)
     lib = pbe.library()
-    opt = pbe.AggOptions(0, 0)
+    opt = pbe.AggOptions(0, 0, 0)
     for A in (pbe.AggregationParams(n=0), pbe.AggregationParams(n=10, kernel_type=6)):
         assert lib.pbe_aggregation_default_options(ctypes.byref(A),
                                                    ctypes.byref(opt)) == INVALID_VALUE

@@ -335,38 +335,43 @@ def test_strips_fine_grid(pbe, oracle_mod, r, kernel):
 
 @pytest.mark.parametrize("kernel", range(6))
 def test_fused_sweep_and_strips(pbe, oracle_mod, kernel):
-    """Mid sizes (at most 32 blocks of 256 columns): the band sweep items and
-    the four-wave strip shares in one launch, for every band split (the
-    launch halves it while the sweep would take over half the slots), on a
-    geometric grid (start-bracket offsets) and a jittered one (searches)."""
+    """Launch form 2 (at most 64 blocks of 256 columns): the band sweep items
+    and the four-wave strip shares in one launch, for every band split (the
+    launch halves it while the sweep would take over half the slots), on
+    geometric grids (start-bracket offsets; r = 1.01 puts near chunks in the
+    strips) and a jittered one (searches); n = 16385 (65 blocks) takes two
+    launches."""
     from pbe_hip import hip, problems as P
-    for n, jitter in ((3400, 0.0), (5000, 0.0), (8192, 0.0), (6000, 0.2)):
-        x = (P.geometric_grid(n, 1e-18, 1.02) if jitter == 0.0
-             else P.jittered_grid(n, 1e-18, 1.02, jitter, seed=n))
+    for n, jitter, r in ((3400, 0.0, 1.02), (5000, 0.0, 1.02), (8192, 0.0, 1.02),
+                         (6000, 0.2, 1.02), (12000, 0.0, 1.01), (16384, 0.0, 1.02),
+                         (16385, 0.0, 1.02)):
+        x = (P.geometric_grid(n, 1e-18, r) if jitter == 0.0
+             else P.jittered_grid(n, 1e-18, r, jitter, seed=n))
         N = P.lognormal_ic(x, x[n // 3], 1.5, 1e12, noise=0.05, seed=60 + kernel)
         log_x0, inv_log_r = P.grid_lookup(x) if jitter == 0.0 else (0.0, 0.0)
         agg = dict(kernel=kernel, beta0=1e-17 if kernel != 2 else 1e-17 / x[n // 3],
                    beta_br=6.73e-18, beta_sh=1e-3, log_x0=log_x0, inv_log_r=inv_log_r)
-        ref, A, D = _oracle(oracle_mod, N, x, agg=agg)
+        ref, A, D = _oracle(oracle_mod, N, x, agg=agg, threads=ORACLE_THREADS)
         dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
         drhs = hip.DeviceArray(n)
         p = pbe.agg_params(n, agg)
         for split in (1, 4, 16):
             drhs.zero()
-            hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=4, band_split=split))
+            hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=4, band_split=split,
+                                                    form=2))
             hip.synchronize()
             got = drhs.download()
-            what = f"fused kernel {kernel} n={n} jitter {jitter} split {split}"
+            what = f"fused kernel {kernel} n={n} r={r} jitter {jitter} split {split}"
             assert_parity(got, ref, A, what=what)
             assert_m1(got, x, A, D, what=what)
 
 
 @pytest.mark.parametrize("n,r,kernel", [(6000, 1.015, 5), (8192, 1.01, 0), (8192, 1.01, 5),
-                                         (12000, 1.015, 3)])
+                                         (12000, 1.015, 3), (16384, 1.005, 5)])
 def test_fine_grid_default_width(pbe, oracle_mod, n, r, kernel):
-    """The default launch on geometric grids whose far-pair reach
-    (282 bins at r = 1.015, 463 at 1.01) exceeds the table's 256-column
-    blocks: the library widens them to 512 columns (no fused launch)."""
+    """The default launch (the tuned table's shape) on geometric grids whose
+    far-pair reach (282 bins at r = 1.015, 463 at 1.01) exceeds 256 columns,
+    where the strips hold near chunks."""
     from pbe_hip import problems as P
     x = P.geometric_grid(n, 1e-18, r)
     N = P.lognormal_ic(x, x[n // 3], 1.5, 1e12, noise=0.05, seed=90 + kernel)
@@ -374,7 +379,7 @@ def test_fine_grid_default_width(pbe, oracle_mod, n, r, kernel):
     agg = dict(kernel=kernel, beta0=1e-17, beta_br=6.73e-18, beta_sh=1e-3, log_x0=log_x0,
                inv_log_r=inv_log_r)
     got = _gpu(pbe, N, x, agg=agg)
-    ref, A, D = _oracle(oracle_mod, N, x, agg=agg)
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg, threads=ORACLE_THREADS)
     assert_parity(got, ref, A, what=f"default width n={n} r={r} kernel {kernel}")
     assert_m1(got, x, A, D, what=f"default width n={n} r={r} kernel {kernel}")
 
@@ -394,12 +399,13 @@ def test_band_every_split(pbe, oracle_mod, kernel):
     dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
     drhs = hip.DeviceArray(n)
     p = pbe.agg_params(n, agg)
-    for u in (2, 4, 8):
+    for u, form in ((2, 1), (4, 1), (4, 2), (8, 1)):
         for split in (1, 2, 4, 8, 16):
             drhs.zero()
-            hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=u, band_split=split))
+            hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=u, band_split=split,
+                                                    form=form))
             hip.synchronize()
             got = drhs.download()
-            what = f"kernel {kernel} U={u} split {split}"
+            what = f"kernel {kernel} U={u} split {split} form {form}"
             assert_parity(got, ref, A, what=what)
             assert_m1(got, x, A, D, what=what)
