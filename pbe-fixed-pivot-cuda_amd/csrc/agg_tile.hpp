@@ -85,6 +85,140 @@ __device__ __forceinline__ void decode_tile(long long t, long long nb, int& p, i
     d = (int)dd;
 }
 
+// Births of a tile's rows by direct brackets, for grids with the geometric
+// lookup (inv_log_r > 0).  On a geometric grid of ratio r the event volume
+// v = x_j + x_k = x_k (1 + r^-e), e = k - j, lies delta(e) =
+// floor(log_r(1 + r^-e)) bins above k whatever k is, so a pair's bracket is
+// k + delta(e): no search, and no dependence on the pair before it (the
+// cursor path below moves a bracket row by row, one LDS round trip after
+// another).  The guess is formed in FP32 with the hardware exp2 / log2 (a
+// few 1e-6 bins of error) and checked against the pivots; a pair whose guess
+// misses (a rounding edge, a grid that is not quite geometric) or falls
+// outside the LDS window takes the exact search.  Rows go kDirectSlots / U
+// at a time, their rates, guesses and pivot reads issued together.  Far
+// pairs (v < x_{k+1}) sum into registers as in the far path; near pairs add
+// their two parts to the LDS window (global atomics beyond it); pairs at or
+// past x_{n-1} clip to bin n-1 (src/aggregation_kernels.cuh:191-192).
+constexpr int kDirectSlots = 4;
+
+template <int KT, int U>
+__device__ __forceinline__ void agg_tile_direct_births(
+    const AggArgs& a, const int k0, const int rw0, const int nrows, const double xlast,
+    const int (&kc)[U], const double (&xk)[U], const double (&xk1)[U], const double (&idx)[U],
+    const double (&Bc)[U][Kern<KT>::R], const double* myrows, AggSmem<KT, U>& sm)
+{
+    constexpr int R = Kern<KT>::R;
+    constexpr int RS = AggSmem<KT, U>::RS;
+    constexpr int WIN = AggShape<U>::WIN;
+    constexpr int G = kDirectSlots / U > 0 ? kDirectSlots / U : 1;   // rows per group
+    const int n = a.n;
+    const BirthSink<WIN> sink{sm.colwin, a.rhs, k0};
+    const float il2r = (float)a.inv_log_r * 0.693147180559945f;   // 1 / log2 r
+    const float l2r = 1.0f / il2r;                                  // log2 r
+    auto xat = [&](int i) { return (i - k0 <= WIN) ? lds_ld(&sm.xw[i - k0]) : glb_ld(&a.x[i]); };
+    auto idat = [&](int i) {
+        const int o = i - k0;
+        return o < WIN ? lds_ld(&sm.idw[o]) : 1.0 / (glb_ld(&a.x[i + 1]) - glb_ld(&a.x[i]));
+    };
+    double cs[U], hw[U], cl[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        cs[u] = 0.0;
+        hw[u] = 0.0;
+        cl[u] = 0.0;
+    }
+    for (int jb = 0; jb < nrows; jb += G) {
+        double pr[G][U], v[G][U], xc[G][U], idc[G][U];
+        int c[G][U], cls[G][U];   // cls 0: no pair, 1: far, 2: near, 3: clipped
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int jj = jb + g;
+            const int j = rw0 + jj;
+            const double* rp = myrows + (jj < nrows ? jj : 0) * RS;
+            double av[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) av[r] = rp[r];
+            const double xj = rp[R];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = kc[u];
+                double p = av[0] * Bc[u][0];
+#pragma unroll
+                for (int r = 1; r < R; ++r) p = fma(av[r], Bc[u][r], p);
+                pr[g][u] = j == k ? 0.5 * p : p;
+                const double vv = xj + xk[u];
+                v[g][u] = vv;
+                const bool has = jj < nrows && j <= k && k < n;
+                cls[g][u] = !has ? 0 : (vv >= xlast ? 3 : (vv < xk1[u] ? 1 : 2));
+                const float d = __builtin_amdgcn_logf(
+                                    1.0f + __builtin_amdgcn_exp2f(-(float)(k - j) * l2r)) * il2r;
+                c[g][u] = max(k + (int)d, k + 1);   // a near pair's bracket is above k
+            }
+        }
+        // the guessed brackets' pivots from the window (index clamped: a
+        // guess past the window counts as a miss)
+        bool miss = false;
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int o = min(c[g][u] - k0, WIN - 1);
+                const double x0 = sm.xw[o], x1 = sm.xw[o + 1];
+                xc[g][u] = x0;
+                idc[g][u] = sm.idw[o];
+                const bool hit = c[g][u] - k0 < WIN && x0 <= v[g][u] && v[g][u] < x1;
+                miss = miss || (cls[g][u] == 2 && !hit);
+            }
+        if (__any(miss)) {
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (cls[g][u] != 2) continue;
+                    const int o = c[g][u] - k0;
+                    if (o < WIN && xc[g][u] <= v[g][u] && v[g][u] < xat(c[g][u] + 1)) continue;
+                    // exact bracket: x_{k+1} <= v < x_{n-1}
+                    const double vv = v[g][u];
+                    int cn;
+                    if (vv < sm.xw[WIN]) {
+                        cn = k0 + window_bracket_lo(sm.xw, kc[u] + 1 - k0, WIN, vv);
+                    } else {
+                        cn = bracket_geometric(a.x, n, vv, a.log_x0, a.inv_log_r) - 1;
+                    }
+                    c[g][u] = cn;
+                    xc[g][u] = xat(cn);
+                    idc[g][u] = idat(cn);
+                }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const double p = pr[g][u];
+                if (cls[g][u] == 3) {
+                    cl[u] += p;
+                } else if (cls[g][u] == 1) {
+                    cs[u] += p;
+                    hw[u] = fma((v[g][u] - xk[u]) * idx[u], p, hw[u]);
+                } else if (cls[g][u] == 2) {
+                    const double h = (v[g][u] - xc[g][u]) * idc[g][u] * p;
+                    sink.put(c[g][u], p - h);
+                    sink.put(c[g][u] + 1, h);
+                }
+            }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int k = kc[u];
+        if (k >= n) continue;
+        if (cs[u] != 0.0 || hw[u] != 0.0) {
+            sink.put(k, cs[u] - hw[u]);
+            sink.put(k + 1, hw[u]);
+        }
+        if (cl[u] != 0.0) sink.put(n - 1, cl[u]);
+    }
+}
+
 // One work item of the pair tile with rows [j0, j0+T) and columns [k0, k0+T)
 // of the upper triangle of the instance described by `a`: the rows
 // [row_base, row_base + 4*rpw), rpw rows per wave (the whole tile, or one
@@ -317,6 +451,9 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
                     lds_add(&colwin[k + 1 - k0], accw[u]);
                 }
             }
+        } else if (a.inv_log_r > 0.0) {
+            agg_tile_direct_births<KT, U>(a, k0, rw0, nrows, xlast, kc, xk, xk1, idx, Bc,
+                                          myrows, sm);
         } else {
             // Cursor sweep.  Rows run upward, so for each column the event
             // volume v = x_j + x_k and its exact bracket (cur, cur+1) only
