@@ -20,8 +20,8 @@ LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(HERE, "_build")
 INCLUDE = os.path.join(REPO, "include")
 
-SOURCES = ["agg_kernels.hip", "brk_kernels.hip", "batch.hip", "integrate.hip", "vjp.hip",
-           "det_kernels.hip", "capi.hip"]
+SOURCES = ["agg_kernels.hip", "brk_kernels.hip", "batch.hip", "integrate.hip", "rhs_fused.hip",
+           "vjp.hip", "det_kernels.hip", "capi.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

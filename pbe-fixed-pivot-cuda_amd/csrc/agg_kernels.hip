@@ -47,19 +47,6 @@
 
 namespace pbe {
 
-// Diagonal band: the 2 nb - 1 tiles with d = 0 (diagonal) and d = 1 (the
-// block below), each split into S row slices; item t -> tile t / S, slice
-// t % S, band tiles in order p = 0.. for d = 0, then for d = 1.
-template <int KT, int U>
-__device__ __forceinline__ void band_item(const AggArgs& a, int t, int S, AggSmem<KT, U>& sm)
-{
-    constexpr int T = AggShape<U>::T, RW = AggShape<U>::RW;
-    const int tt = t / S, sl = t - tt * S;
-    const int d = tt < a.nb ? 0 : 1;
-    const int p = tt < a.nb ? tt : tt - a.nb;
-    agg_tile_body<KT, U>(a, p * T, (p + d) * T, d == 0, p * T + sl * (T / S), RW / S, sm);
-}
-
 template <int KT, int U>
 __global__ __launch_bounds__(256, (U >= 8 ? 2 : 3))
 void agg_band_kernel(AggArgs a, int S)
@@ -158,19 +145,10 @@ static const BandSegs& band_segments_cached(int S, double inv_log_r)
 // diagonal-major order, all scheduled by the hardware.
 template <int KT, int U>
 __global__ __launch_bounds__(256, (U >= 8 ? 2 : 3))
-void agg_tiles_kernel(AggArgs a, int S)
+void agg_tiles_kernel(AggArgs a, int S, int D)
 {
     __shared__ AggSmem<KT, U> sm;
-    constexpr int T = AggShape<U>::T, RW = AggShape<U>::RW;
-    const long long bt = band_tiles(a.nb);
-    const long long t = blockIdx.x;
-    if (t < S * bt) {
-        band_item<KT, U>(a, (int)t, S, sm);
-    } else {
-        int p, d;
-        decode_tile(t - S * bt + bt, a.nb, p, d);
-        agg_tile_body<KT, U>(a, p * T, (p + d) * T, d == 0, p * T, RW, sm);
-    }
+    agg_tiles_item<KT, U>(a, blockIdx.x, S, D, sm);
 }
 
 // Column strips over the resident grid (with or without each block's
@@ -318,48 +296,98 @@ static StripPlan make_strip_plan(int n, int U, bool band, double inv_log_r)
     return plan;
 }
 
+// Which launch(es) an aggregation call makes at block width U (the tuned
+// band split and form given): the fused band + strip launch, one launch of
+// every tile, or a band launch followed by the strip launch.
+enum class AggForm { Fused, Tiles, BandStrips };
+struct AggLaunch {
+    AggForm form;
+    int nseg;            // Fused: offset segments per block
+    long long groups;    // Fused / BandStrips: resident workgroups of the strip part
+    long long items;     // Tiles: work items; Fused: sweep items; BandStrips: band items
+    int diags;           // Tiles: leading diagonals run as row slices
+    StripPlan sep;       // the strips below each block's band
+};
+
+// Diagonals of T-column tiles that hold near pairs: a tile on diagonal d
+// has offsets k - j >= (d - 1) T + 1, so with near-pair reach `reach` those
+// with d <= 1 + (reach - 1) / T (at least the two band diagonals).
+static int tile_band_diags(int T, double inv_log_r)
+{
+    const double reach = grid_reach(inv_log_r);
+    if (!(reach > T)) return 2;
+    const double d = 2.0 + std::floor((reach - 1.0) / T);
+    return d < 64.0 ? (int)d : 64;
+}
+
+template <int KT, int U>
+static AggLaunch plan_agg_t(int n, double inv_log_r, int split, bool fused)
+{
+    constexpr int T = AggShape<U>::T;
+    const int nb = (n + T - 1) / T;
+    AggLaunch L;
+    L.sep = make_strip_plan(n, U, false, inv_log_r);
+    L.nseg = split;
+    if (fused && T == kBandT && nb <= kFusedMaxBlocks && L.sep.total > 0) {
+        // mid sizes: sweep items on at most half of the resident slots of
+        // the fused kernel, strip shares on the rest (empty shares return)
+        const long long Gf = fused_resident_groups<KT>();
+        int Sf = split;
+        while (Sf > 1 && (long long)nb * Sf * 2 > Gf) Sf /= 2;
+        const long long nsweep = (long long)nb * Sf;
+        if (nsweep * 2 <= Gf) {
+            L.form = AggForm::Fused;
+            L.nseg = Sf;
+            L.groups = Gf;
+            L.items = nsweep;
+            return L;
+        }
+    }
+    L.groups = resident_groups<KT, U>();
+    if (L.sep.total < L.groups) {
+        // too little far work for the strips: one launch of all tiles
+        const long long tiles = (long long)nb * (nb + 1) / 2;
+        L.form = AggForm::Tiles;
+        L.diags = tile_band_diags(T, inv_log_r);
+        const long long bt = band_tiles_d(nb, L.diags);
+        L.items = (long long)split * bt + tiles - bt;
+        return L;
+    }
+    L.form = AggForm::BandStrips;
+    L.items = (long long)split * band_tiles(nb);
+    return L;
+}
+
 template <int KT, int U>
 static hipError_t launch_agg_t(AggArgs args, int split, bool fused, hipStream_t stream)
 {
     constexpr int T = AggShape<U>::T;
     args.nb = (args.n + T - 1) / T;
-    const long long band_items = (long long)split * band_tiles(args.nb);
-    const StripPlan sep = make_strip_plan(args.n, U, false, args.inv_log_r);
-    if (fused && T == kBandT && args.nb <= kFusedMaxBlocks && sep.total > 0) {
-        // mid sizes: sweep items on at most half of the resident slots of
-        // the fused kernel, strip shares on the rest (empty shares return)
-        const long long Gf = fused_resident_groups<KT>();
-        int Sf = split;
-        while (Sf > 1 && (long long)args.nb * Sf * 2 > Gf) Sf /= 2;
-        const long long nsweep = (long long)args.nb * Sf;
-        if (nsweep * 2 <= Gf) {
-            hipLaunchKernelGGL((agg_fused_kernel<KT>), dim3((unsigned)Gf),
-                               dim3(kWave * kFusedWaves), 0, stream, args, Sf,
-                               band_segments_cached(Sf, args.inv_log_r), sep, (int)nsweep);
-            return hipSuccess;
-        }
-    }
-    const long long G = resident_groups<KT, U>();
-    if (sep.total < G) {
-        // too little far work for the strips: one launch of all tiles
-        const long long tiles = (long long)args.nb * (args.nb + 1) / 2;
-        const long long items = band_items + tiles - band_tiles(args.nb);
-        if (items > 0x7fffffffLL) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((agg_tiles_kernel<KT, U>), dim3((unsigned)items), dim3(256), 0,
-                           stream, args, split);
+    const AggLaunch L = plan_agg_t<KT, U>(args.n, args.inv_log_r, split, fused);
+    if (L.items > 0x7fffffffLL) return hipErrorInvalidValue;
+    switch (L.form) {
+    case AggForm::Fused:
+        hipLaunchKernelGGL((agg_fused_kernel<KT>), dim3((unsigned)L.groups),
+                           dim3(kWave * kFusedWaves), 0, stream, args, L.nseg,
+                           band_segments_cached(L.nseg, args.inv_log_r), L.sep, (int)L.items);
         return hipSuccess;
+    case AggForm::Tiles:
+        hipLaunchKernelGGL((agg_tiles_kernel<KT, U>), dim3((unsigned)L.items), dim3(256), 0,
+                           stream, args, split, L.diags);
+        return hipSuccess;
+    case AggForm::BandStrips:
+        break;
     }
-    if (band_items > 0x7fffffffLL) return hipErrorInvalidValue;
     if (T == kBandT) {
         // band as diagonal sweeps: split = offset segments per block
         hipLaunchKernelGGL((agg_sweep_kernel<KT>), dim3((unsigned)(args.nb * split)), dim3(256),
                            0, stream, args, split, band_segments_cached(split, args.inv_log_r));
     } else {
-        hipLaunchKernelGGL((agg_band_kernel<KT, U>), dim3((unsigned)band_items), dim3(256), 0,
+        hipLaunchKernelGGL((agg_band_kernel<KT, U>), dim3((unsigned)L.items), dim3(256), 0,
                            stream, args, split);
     }
-    hipLaunchKernelGGL((agg_strip_kernel<KT, U>), dim3((unsigned)G),
-                       dim3(kWave * StripWaves<U>::value), 0, stream, args, sep);
+    hipLaunchKernelGGL((agg_strip_kernel<KT, U>), dim3((unsigned)L.groups),
+                       dim3(kWave * StripWaves<U>::value), 0, stream, args, L.sep);
     return hipSuccess;
 }
 
@@ -391,14 +419,43 @@ hipError_t launch_aggregation(const double* N, const double* x, double* rhs,
     args.coef = KernelCoef{p.beta0, p.beta_br, p.beta_sh};
     args.log_x0 = p.log_x0;
     args.inv_log_r = p.inv_log_r;
-    const int S = (band_split == 1 || band_split == 2 || band_split == 8 || band_split == 16)
-        ? band_split : 4;
+    const int S = band_split_or_default(band_split);
     const bool fused = form == 2;
     switch (tile_u) {
     case 1: return launch_agg_u<1>(args, p.kernel_type, S, fused, stream);
     case 2: return launch_agg_u<2>(args, p.kernel_type, S, fused, stream);
     case 8: return launch_agg_u<8>(args, p.kernel_type, S, fused, stream);
     default: return launch_agg_u<4>(args, p.kernel_type, S, fused, stream);
+    }
+}
+
+template <int U>
+static TilesShape tiles_shape_u(int n, int kernel, double inv_log_r, int split, bool fused)
+{
+    AggLaunch L;
+    switch (kernel) {
+    case 0: L = plan_agg_t<0, U>(n, inv_log_r, split, fused); break;
+    case 1: L = plan_agg_t<1, U>(n, inv_log_r, split, fused); break;
+    case 2: L = plan_agg_t<2, U>(n, inv_log_r, split, fused); break;
+    case 3: L = plan_agg_t<3, U>(n, inv_log_r, split, fused); break;
+    case 4: L = plan_agg_t<4, U>(n, inv_log_r, split, fused); break;
+    case 5: L = plan_agg_t<5, U>(n, inv_log_r, split, fused); break;
+    default: return TilesShape{-1, 2};
+    }
+    return L.form == AggForm::Tiles ? TilesShape{L.items, L.diags} : TilesShape{-1, 2};
+}
+
+TilesShape aggregation_tiles_shape(const pbe_aggregation_params& p, int tile_u, int band_split,
+                                   int form)
+{
+    const int S = band_split_or_default(band_split);
+    const bool fused = form == 2;
+    switch (tile_u) {
+    case 1: return tiles_shape_u<1>(p.n, p.kernel_type, p.inv_log_r, S, fused);
+    case 2: return tiles_shape_u<2>(p.n, p.kernel_type, p.inv_log_r, S, fused);
+    case 4: return tiles_shape_u<4>(p.n, p.kernel_type, p.inv_log_r, S, fused);
+    case 8: return tiles_shape_u<8>(p.n, p.kernel_type, p.inv_log_r, S, fused);
+    default: return TilesShape{-1, 2};
     }
 }
 

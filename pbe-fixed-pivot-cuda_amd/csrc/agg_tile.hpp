@@ -37,9 +37,15 @@ __host__ __device__ __forceinline__ long long band_tiles(long long nb)
 }
 
 // Tiles before diagonal d in diagonal-major order.
-__device__ __forceinline__ long long tiles_before(long long d, long long nb)
+__host__ __device__ __forceinline__ long long tiles_before(long long d, long long nb)
 {
     return d * nb - d * (d - 1) / 2;
+}
+
+// Tiles of the first D diagonals (d = 0 .. D-1) of an nb-block triangle.
+__host__ __device__ __forceinline__ long long band_tiles_d(long long nb, int D)
+{
+    return tiles_before(D < nb ? D : nb, nb);
 }
 
 template <int WIN>
@@ -345,8 +351,8 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
                 xk1[u] = sm.xw[c + 1];
                 idx[u] = sm.idw[c];
             } else {
-                xk1[u] = xv;     // never "far": every pair of column n-1 clips
-                idx[u] = 0.0;
+                xk1[u] = xv;     // every pair of column n-1 clips to bin n-1
+                idx[u] = 0.0;    // (weight 0 in the far loop)
             }
         } else {
 #pragma unroll
@@ -406,16 +412,20 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
     // ---- births of this wave's pairs ----
     if (rw0 < rw1) {
         const int jmax = rw1 - 1;
-        const double xjmax = a.x[jmax];
+        const double xjmax = myrows[(jmax - rw0) * RS + R];   // this wave's last row
+        // every pair far (v < x_{k+1}), or in column n-1, whose pairs all
+        // clip to bin n-1 (src/aggregation_kernels.cuh:191-192): there the
+        // far loop's weight factor 1/(x_{k+1} - x_k) is 0, so the column's
+        // whole rate goes to bin n-1
         bool ok = true;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int k = kc[u];
-            ok = ok && (k >= n || (k <= n - 2 && jmax < k && (xjmax + xk[u]) < xk1[u]));
+            ok = ok && (k >= n || (jmax < k && (k == n - 1 || (xjmax + xk[u]) < xk1[u])));
         }
         const int nrows = rw1 - rw0;
         if (__all(ok)) {
-            // every pair lands in (k, k+1): registers only
+            // every pair lands in (k, k+1) or clips: registers only
             double accw[U], colsum[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -565,5 +575,40 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
     }
 }
 
+
+// Diagonal band: the 2 nb - 1 tiles with d = 0 (diagonal) and d = 1 (the
+// block below), each split into S row slices; item t -> tile t / S, slice
+// t % S, band tiles in order p = 0.. for d = 0, then for d = 1.
+template <int KT, int U>
+__device__ __forceinline__ void band_item(const AggArgs& a, int t, int S, AggSmem<KT, U>& sm)
+{
+    constexpr int T = AggShape<U>::T, RW = AggShape<U>::RW;
+    const int tt = t / S, sl = t - tt * S;
+    const int d = tt < a.nb ? 0 : 1;
+    const int p = tt < a.nb ? tt : tt - a.nb;
+    agg_tile_body<KT, U>(a, p * T, (p + d) * T, d == 0, p * T + sl * (T / S), RW / S, sm);
+}
+
+// Item t of the single tile launch: the tiles of the first D diagonals
+// (those holding near pairs: D = 2 on grids whose near-pair reach is within
+// a tile) as S row slices each, first, then the other tiles whole, all in
+// diagonal-major order.
+template <int KT, int U>
+__device__ __forceinline__ void agg_tiles_item(const AggArgs& a, long long t, int S, int D,
+                                               AggSmem<KT, U>& sm)
+{
+    constexpr int T = AggShape<U>::T, RW = AggShape<U>::RW;
+    const long long bt = band_tiles_d(a.nb, D);
+    int p, d;
+    if (t < S * bt) {
+        const long long tt = t / S;
+        const int sl = (int)(t - tt * S);
+        decode_tile(tt, a.nb, p, d);
+        agg_tile_body<KT, U>(a, p * T, (p + d) * T, d == 0, p * T + sl * (T / S), RW / S, sm);
+    } else {
+        decode_tile(t - S * bt + bt, a.nb, p, d);
+        agg_tile_body<KT, U>(a, p * T, (p + d) * T, d == 0, p * T, RW, sm);
+    }
+}
 
 }  // namespace pbe

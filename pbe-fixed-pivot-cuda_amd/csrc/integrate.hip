@@ -9,7 +9,8 @@
 //   N    += (dt/6) * (((k1 + 2 k2) + 2 k3) + k4)
 // (no FMA contraction, so the rounding is the reference's).  The stage kernel
 // that consumes k also zeroes it for the next evaluation, so no memset sits in
-// the sequence and the whole step can be captured into a hipGraph.
+// the sequence and the whole step can be captured into a hipGraph.  Small
+// problems evaluate the right-hand side in one launch (rhs_fused.hip).
 #include "pbe_device.hpp"
 #include "pbe_internal.hpp"
 
@@ -70,6 +71,10 @@ hipError_t enqueue_rhs(const RhsProblem& p, const double* Nin, double* rhs, hipS
 {
     hipError_t e = hipSuccess;
     if (p.batch) return enqueue_batched_rhs(p.batch, Nin, p.x, p.t_q, p.bw_q, rhs, s);
+    // small problems: aggregation tiles and breakage blocks in one launch
+    e = launch_rhs_fused(p, Nin, rhs, s);
+    if (e != hipErrorNotSupported) return e;
+    e = hipSuccess;
     if (p.agg) {
         const TileChoice t = tile_choice(p.agg->n, p.agg->kernel_type, p.agg->inv_log_r);
         e = launch_aggregation(Nin, p.x, rhs, *p.agg, t.u, t.w, t.f, s);

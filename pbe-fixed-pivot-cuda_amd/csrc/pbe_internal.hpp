@@ -19,6 +19,28 @@ hipError_t launch_breakage(const double* N, const double* x, const double* t_q,
                            const double* bw_q, double* rhs,
                            const pbe_breakage_params& p, hipStream_t stream);
 
+// Band split the launchers use for a requested one (1, 2, 4, 8, 16; else 4).
+inline int band_split_or_default(int s)
+{
+    return (s == 1 || s == 2 || s == 8 || s == 16) ? s : 4;
+}
+
+// The single tile launch (agg_tiles_kernel) an aggregation call at this
+// shape makes: its work items and the leading tile diagonals run as row
+// slices; items = -1 when the call makes other launches (strips, the fused
+// band + strip launch).
+struct TilesShape { long long items; int diags; };
+TilesShape aggregation_tiles_shape(const pbe_aggregation_params& p, int tile_u, int band_split,
+                                   int form);
+
+// R_agg(N) + R_brk(N) added into rhs in one launch (rhs_fused.hip), for
+// problems with both terms whose aggregation is a single tile launch.
+// Returns hipErrorNotSupported, enqueueing nothing, for any other problem
+// (the caller then makes the separate launches).
+struct RhsProblem;
+hipError_t launch_rhs_fused(const RhsProblem& p, const double* N, double* rhs,
+                            hipStream_t stream);
+
 // Launch shape of the aggregation kernels for a problem (the autotuned
 // table, capi.hip): columns per lane, band-tile row slices and launch form.
 struct TileChoice { int u, w, f; };

@@ -76,6 +76,47 @@ def test_flocculation_rk4_vs_oracle(pbe, oracle_mod):
     assert err <= 1e-11
 
 
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("selection", [0, 1, 2, 3])
+def test_fused_rhs_every_kernel_and_selection(pbe, oracle_mod, kernel, selection):
+    """Small problems evaluate R_agg + R_brk in one launch (aggregation tile
+    items and breakage parent blocks in one grid): every collision kernel x
+    selection function under the integrator against the oracle's RK4 and
+    Euler, on a geometric (n = 200) and a jittered (n = 333, binary-search
+    brackets) grid, with an aggregation-only and a breakage-only problem
+    (separate launches) beside it."""
+    from pbe_hip import hip, problems as P
+    import test_gpu_adjoint as TA
+    x, N0, agg, brk, _, _ = TA._small_problem(200, kernel, selection)
+    t_q, bw_q = P.daughter_uniform(8)
+    xj = P.jittered_grid(333, 1e-18, 1.03, 0.2, seed=kernel * 4 + selection)
+    Nj = P.lognormal_ic(xj, xj[200], 0.8, 1e4, noise=0.05, seed=3)
+    aj = dict(agg, log_x0=0.0, inv_log_r=0.0)
+    bj = dict(brk, v_ref=float(xj[166]), v_min=float(xj[166]))
+    cases = [(x, N0, agg, brk), (xj, Nj, aj, bj), (x, N0, agg, None), (x, N0, None, brk)]
+    for ci, (xx, NN, a, b) in enumerate(cases):
+        kw = dict(agg=a, brk=b, t_q=t_q if b else None, bw_q=bw_q if b else None)
+        for method, steps in ((pbe.METHOD_RK4, 6), (pbe.METHOD_EULER, 3)):
+            it = pbe.Integrator(xx, method=method, **kw)
+            dN = hip.DeviceArray.from_host(NN)
+            hip.check(it.step(dN, 0.5, steps))
+            hip.synchronize()
+            got = dN.download()
+            if method == pbe.METHOD_RK4:
+                ref = oracle_mod.rk4(NN, xx, 0.5, steps, **kw)
+            else:
+                ref = NN.copy()
+                for _ in range(steps):
+                    r = np.zeros_like(ref)
+                    if a:
+                        r += oracle_mod.aggregation_rhs(ref, xx, **a)[0]
+                    if b:
+                        r += oracle_mod.breakage_rhs(ref, xx, t_q, bw_q, **b)[0]
+                    ref = ref + 0.5 * r
+            err = np.abs(got - ref).max() / np.abs(ref).max()
+            assert err <= 1e-11, (ci, method, err)
+
+
 def test_euler_vs_oracle_one_step(pbe, oracle_mod):
     from pbe_hip import hip, problems as P
     x, N0, agg, brk, t_q, bw_q = P.config_flocculation(300)
