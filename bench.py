@@ -142,6 +142,43 @@ def adjoint_form(pbe, hip, P):
     return out
 
 
+def batched_fit_form(hip, P):
+    """The adjoint workload's batched variant: 256 instances, n log-uniform
+    in [128, 1024], each with its own G and initial PSD, fitted jointly.
+    Times one forward solve and one gradient (batched adjoint forward +
+    reverse, checkpoints every 50) at 2000 RK4 steps of 0.5 s, and runs an
+    L-BFGS fit of the common factors at 200 steps from all factors 1 to a
+    target solved at known ones."""
+    from pbe_hip.fit import BatchedFit
+    inst, N0, x, t_q, bw_q = P.config_batched_fit(256, seed=256)
+    rng = np.random.default_rng(11)
+    th_true = np.array([*np.log(rng.uniform(0.5, 2.0, 3)), (1.0 / 3.0) * rng.uniform(0.5, 2.0)])
+    out = {"instances": len(inst), "bins": len(N0),
+           "pairs_per_eval": sum(d["n"] * (d["n"] + 1) // 2 for d in inst),
+           "workload": ("256 instances, n log-uniform [128, 1024], r = 1.03, own G in [0.5, 2], "
+                        "Brownian+shear + power-law breakage, binary daughters; fitted: common "
+                        "factors of beta_br, beta_sh, S0 and the exponent alpha")}
+    F = BatchedFit(inst, N0, x, t_q, bw_q, 0.5, 2000, every=50)
+    t0 = time.perf_counter()
+    F.set_target(th_true)
+    t1 = time.perf_counter()
+    F.loss_and_grad(F.theta0)
+    t2 = time.perf_counter()
+    out["steps"] = 2000
+    out["forward_solve_s"] = t1 - t0
+    out["gradient_s"] = t2 - t1
+    out["gradient_over_forward"] = (t2 - t1) / (t1 - t0)
+    G = BatchedFit(inst, N0, x, t_q, bw_q, 0.5, 200, every=50)
+    G.set_target(th_true)
+    t3 = time.perf_counter()
+    th, hist, res = G.fit(max_evals=60)
+    t4 = time.perf_counter()
+    out["fit"] = {"steps": 200, "evaluations": len(hist), "seconds": t4 - t3,
+                  "loss_start": hist[0], "loss_end": hist[-1],
+                  "max_abs_theta_error": float(np.max(np.abs(th - th_true)))}
+    return out
+
+
 def rhs_variant(pbe_hip, hip, P, n, daughter, steps=20, r=1.02):
     """Average step time (zero rhs, aggregation, breakage) of one flocculation
     RHS configuration, HIP events on the launch stream."""
@@ -435,6 +472,8 @@ def main():
     if not args.no_variants:
         out["variants"] = [rhs_variant(pbe_hip, hip, P, m, "symmetric") for m in (1024, 4096)]
         out["variants"].append(rhs_variant(pbe_hip, hip, P, n, "uniform"))
+        # a finer grid: near-pair reach 1062 bins
+        out["variants"].append(rhs_variant(pbe_hip, hip, P, n, "symmetric", r=1.005))
         # past the int32 pair count (2.1e9 pairs); r = 1.005 keeps x finite
         out["variants"].append(rhs_variant(pbe_hip, hip, P, 65536, "symmetric", steps=5,
                                            r=1.005))
@@ -444,6 +483,7 @@ def main():
         out["forms"]["deterministic"]["over_atomic"] = (
             out["forms"]["deterministic"]["ms_per_step"] / ms_step)
         out["forms"]["flocculation_two_stage"] = flocculation_form()
+        out["forms"]["batched_fit"] = batched_fit_form(hip, P)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, x, agg, brk, t_q, bw_q, args.cpu_seconds)
     print(json.dumps(out))
