@@ -279,6 +279,32 @@ def test_every_tile_shape(pbe, oracle_mod, u, n, kernel):
     assert_m1(got, x, A, D, what=f"U={u} n={n}")
 
 
+@pytest.mark.parametrize("r", [1.005, 1.002])
+@pytest.mark.parametrize("u,n", [(1, 384), (1, 1500), (2, 1500), (4, 900)])
+@pytest.mark.parametrize("kernel", [0, 5])
+def test_tiles_sliced_near_diagonals(pbe, oracle_mod, r, u, n, kernel):
+    """The single tile launch on grids whose near-pair reach spans many tiles
+    (r = 1.005: 1062 bins, r = 1.002: 3107): every tile diagonal within the
+    reach runs as row slices, the rest whole, column n-1 through the far
+    loop."""
+    from pbe_hip import hip, problems as P
+    x = P.geometric_grid(n, 1e-18, r)
+    N = P.lognormal_ic(x, x[n // 2], 1.5, 1e12, noise=0.05, seed=u + n)
+    log_x0, inv_log_r = P.grid_lookup(x)
+    agg = dict(kernel=kernel, beta0=1e-17, beta_br=6.73e-18, beta_sh=1e-3, log_x0=log_x0,
+               inv_log_r=inv_log_r)
+    dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
+    drhs = hip.DeviceArray(n)
+    drhs.zero()
+    p = pbe.agg_params(n, agg)
+    hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=u, band_split=8, form=1))
+    hip.synchronize()
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg)
+    got = drhs.download()
+    assert_parity(got, ref, A, what=f"U={u} n={n} r={r}")
+    assert_m1(got, x, A, D, what=f"U={u} n={n} r={r}")
+
+
 def test_strips_jittered_grid(pbe, oracle_mod):
     """Large non-geometric grid (r = 1.01 +-20 %): the column-strip launch with
     the binary-search bracket and near pairs below the diagonal band."""
