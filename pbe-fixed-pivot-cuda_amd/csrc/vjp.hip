@@ -40,6 +40,7 @@ struct VjpArgs {
     double* gth_rows;                 // += per-row parameter partials [n][kNP]
     int n, nq;
     KernelCoef coef;
+    double log_x0, inv_log_r;         // geometric bracket lookup (inv_log_r = 0: search)
     int sel;                          // -1: no breakage
     double S0, v_ref, alpha, v_min;
     int log_scale;                    // 1: slots 0-3 hold d/d ln(coefficient)
@@ -175,13 +176,17 @@ __device__ __forceinline__ void vjp_part_body(const VjpArgs& a, const int m0, co
         const double xm = a.x[m], lm = a.lam[m];
         const double cm = a.tab[4 * m + 0], icm = a.tab[4 * m + 1];
         const double xlast = a.x[n - 1], llast = a.lam[n - 1];
-        // bracket of the first pair of this quarter
+        // bracket of the first pair of this quarter: on a geometric grid the
+        // log-ratio guess with its exact fix-up (a few loads), else a binary
+        // search
         int cur = -1;
         double xc = 0.0, xc1 = 0.0, lc = 0.0, lc1 = 0.0, idc = 0.0;
         {
             const double v0 = xm + a.x[wb];
             if (v0 < xlast) {
-                cur = bracket_search(a.x, max(m, wb), n - 1, v0) - 1;
+                cur = (a.inv_log_r != 0.0
+                           ? bracket_geometric(a.x, n, v0, a.log_x0, a.inv_log_r)
+                           : bracket_search(a.x, max(m, wb), n - 1, v0)) - 1;
                 xc = a.x[cur];
                 xc1 = a.x[cur + 1];
                 lc = a.lam[cur];
@@ -189,12 +194,22 @@ __device__ __forceinline__ void vjp_part_body(const VjpArgs& a, const int m0, co
                 idc = a.tab[4 * cur + 2];
             }
         }
+        // column data of the next pair loaded before this pair's use (the
+        // index clamped into the quarter, so the loads are unconditional)
+        auto colload = [&](int k, double& xk, double& Nk, double& lk, double& ck,
+                           double& ick) {
+            const int kk = min(k, we - 1);
+            xk = a.x[kk];
+            Nk = a.N[kk];
+            lk = a.lam[kk];
+            ck = a.tab[4 * kk + 0];
+            ick = a.tab[4 * kk + 1];
+        };
+        double xn, Nn, ln, cn, icn;
+        colload(wb, xn, Nn, ln, cn, icn);
         for (int k = wb; k < we; ++k) {
-            const double xk = a.x[k];
-            const double Nk = a.N[k];
-            const double lk = a.lam[k];
-            const double ck = a.tab[4 * k + 0];
-            const double ick = a.tab[4 * k + 1];
+            const double xk = xn, Nk = Nn, lk = ln, ck = cn, ick = icn;
+            colload(k + 1, xn, Nn, ln, cn, icn);
             const double v = xm + xk;
             double eta;
             if (v >= xlast) {
@@ -207,6 +222,133 @@ __device__ __forceinline__ void vjp_part_body(const VjpArgs& a, const int m0, co
                     xc1 = a.x[cur + 1];
                     lc1 = a.lam[cur + 1];
                     idc = a.tab[4 * cur + 2];
+                }
+                const double w = (v - xc) * idc;
+                eta = fma(w, lc1 - lc, lc);
+            }
+            const double t = Nk * ((eta - lm) - lk);
+            double f[G];
+            VG::phi(f, xm, cm, icm, xk, ck, ick);
+#pragma unroll
+            for (int g = 0; g < G; ++g) acc[g] = fma(f[g], t, acc[g]);
+        }
+    }
+    if (wv > 0) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) sm.part[wv - 1][lane][g] = acc[g];
+    }
+    __syncthreads();
+    if (wv == 0 && row) {
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            out[(size_t)G * m + g] =
+                ((acc[g] + sm.part[0][lane][g]) + sm.part[1][lane][g]) + sm.part[2][lane][g];
+    }
+}
+
+// Columns per chunk of the single-instance part launch (its LDS staging
+// holds them) and bracket bins staged beyond the chunk.
+constexpr int kVjpMaxCols = 256;
+constexpr int kVjpWinExtra = 64;
+constexpr int kVjpWin = kVjpMaxCols + kVjpWinExtra + 1;
+
+struct VjpStage {
+    double wx[kVjpWin], wl[kVjpWin], wid[kVjpWin];   // bins kb..: x, lam, 1/(x_{i+1}-x_i)
+    double cn[kVjpMaxCols], cc[kVjpMaxCols], cic[kVjpMaxCols];   // columns: N, c, 1/c
+};
+
+// vjp_part_body with the chunk's column data and the bins just above it
+// staged in LDS by the whole workgroup first (ke - kb <= kVjpMaxCols): the
+// pair loop then reads a column as LDS broadcasts, and a bracket move within
+// the staged bins reads LDS too (beyond them, global memory).
+template <int KT>
+__device__ __forceinline__ void vjp_part_body_staged(const VjpArgs& a, const int m0,
+                                                     const int kb, const int ke,
+                                                     double* __restrict__ out,
+                                                     VjpSmem<KT>& sm, VjpStage& st)
+{
+    constexpr int G = VjpSmem<KT>::G;
+    using VG = VjpGroups<(KT >= 0 ? KT : 0)>;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int n = a.n;
+    const int m = m0 + lane;
+    const bool row = m < n;
+    const int nw = min(ke - kb + kVjpWinExtra + 1, n - kb);   // staged bins
+    for (int i = tid; i < nw; i += blockDim.x) {
+        const int b = kb + i;
+        st.wx[i] = a.x[b];
+        st.wl[i] = a.lam[b];
+        st.wid[i] = a.tab[4 * b + 2];
+    }
+    for (int i = tid; i < ke - kb; i += blockDim.x) {
+        const int k = kb + i;
+        st.cn[i] = a.N[k];
+        st.cc[i] = a.tab[4 * k + 0];
+        st.cic[i] = a.tab[4 * k + 1];
+    }
+    double acc[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) acc[g] = 0.0;
+    const int q = (ke - kb + 3) / 4;
+    const int wb = kb + wv * q, we = min(ke, wb + q);
+    double xm = 0.0, lm = 0.0, cm = 0.0, icm = 0.0, xlast = 0.0, llast = 0.0;
+    if (row) {
+        xm = a.x[m];
+        lm = a.lam[m];
+        cm = a.tab[4 * m + 0];
+        icm = a.tab[4 * m + 1];
+        xlast = a.x[n - 1];
+        llast = a.lam[n - 1];
+    }
+    __syncthreads();
+    if (KT >= 0 && row && wb < we) {
+        auto xat = [&](int i) { return i - kb < nw ? lds_ld(&st.wx[i - kb]) : glb_ld(&a.x[i]); };
+        auto lat = [&](int i) { return i - kb < nw ? lds_ld(&st.wl[i - kb]) : glb_ld(&a.lam[i]); };
+        auto iat = [&](int i) {
+            return i - kb < nw ? lds_ld(&st.wid[i - kb]) : glb_ld(&a.tab[4 * i + 2]);
+        };
+        // bracket of the first pair of this quarter (cur >= wb: v >= x_wb)
+        int cur = -1;
+        double xc = 0.0, xc1 = 0.0, lc = 0.0, lc1 = 0.0, idc = 0.0;
+        {
+            const double v0 = xm + st.wx[wb - kb];
+            if (v0 < xlast) {
+                int hi;
+                if (v0 < st.wx[nw - 1]) {
+                    int lo = wb - kb, h = nw - 1;   // wx[lo] <= v0 < wx[h]
+                    while (h - lo > 1) {
+                        const int mid = (lo + h) >> 1;
+                        if (st.wx[mid] <= v0) lo = mid; else h = mid;
+                    }
+                    hi = kb + h;
+                } else {
+                    hi = a.inv_log_r != 0.0 ? bracket_geometric(a.x, n, v0, a.log_x0, a.inv_log_r)
+                                            : bracket_search(a.x, kb + nw - 1, n - 1, v0);
+                }
+                cur = hi - 1;
+                xc = xat(cur);
+                xc1 = xat(cur + 1);
+                lc = lat(cur);
+                lc1 = lat(cur + 1);
+                idc = iat(cur);
+            }
+        }
+        for (int k = wb; k < we; ++k) {
+            const int o = k - kb;
+            const double xk = st.wx[o], Nk = st.cn[o], lk = st.wl[o];
+            const double ck = st.cc[o], ick = st.cic[o];
+            const double v = xm + xk;
+            double eta;
+            if (v >= xlast) {
+                eta = llast;                     // clipped to the last pivot
+            } else {
+                while (v >= xc1) {
+                    ++cur;
+                    xc = xc1;
+                    lc = lc1;
+                    xc1 = xat(cur + 1);
+                    lc1 = lat(cur + 1);
+                    idc = iat(cur);
                 }
                 const double w = (v - xc) * idc;
                 eta = fma(w, lc1 - lc, lc);
@@ -321,7 +463,10 @@ __device__ __forceinline__ double vjp_brk_gather_wave(const VjpArgs& a, const in
         if (v <= a.x[0]) et = a.lam[0];
         else if (v >= a.x[n - 1]) et = a.lam[n - 1];
         else {
-            const int hi = bracket_search(a.x, 0, v < xmh ? mh : n - 1, v), lo = hi - 1;
+            const int hi = a.inv_log_r != 0.0
+                ? bracket_geometric(a.x, n, v, a.log_x0, a.inv_log_r)
+                : bracket_search(a.x, 0, v < xmh ? mh : n - 1, v);
+            const int lo = hi - 1;
             const double xl = a.x[lo], xh = a.x[hi];
             const double w = (v - xl) / (xh - xl);
             et = (1.0 - w) * a.lam[lo] + w * a.lam[hi];
@@ -333,18 +478,31 @@ __device__ __forceinline__ double vjp_brk_gather_wave(const VjpArgs& a, const in
 
 // Single instance: one wave per row.  Lane c adds the column-chunk partials
 // c, c + 64, ..., summed over the wave in wave_sum's fixed order; the
-// breakage gather of the row comes precomputed in brk_e.
+// breakage terms of the row come precomputed in brk[4m..4m+3] (the gather
+// sum_q bw_q eta(x_m t_q), S_m, dS_m/dS0 in the gradient's scale, and
+// S_m ln(x_m / v_ref)).  Every load of the row, the read-modify-writes
+// included, is issued before the sums that need them.
 template <int KT>
 __device__ __forceinline__ void vjp_finish_row_wave(const VjpArgs& a, const int m,
                                                     const double* __restrict__ part,
                                                     int nchunk, size_t chunk_stride,
-                                                    const double* __restrict__ brk_e)
+                                                    const double* __restrict__ brk)
 {
     constexpr int G = VjpSmem<KT>::G;
     const int lane = threadIdx.x & (kWave - 1);
-    const double Nm = a.N[m], lm = a.lam[m], xm = a.x[m];
+    const double Nm = a.N[m], lm = a.lam[m];
+    double* grow = a.gth_rows + (size_t)kNP * m;
+    double gr[kNP], lacc = 0.0, lbase = 0.0;
+#pragma unroll
+    for (int p = 0; p < kNP; ++p) gr[p] = grow[p];
+    if (a.lam_acc) lacc = a.lam_acc[m];
+    if (a.av_next) lbase = a.lam_base[m];
+    double b4[4] = {0.0, 0.0, 0.0, 0.0};
+    if (a.sel >= 0) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) b4[t] = brk[4 * (size_t)m + t];
+    }
     double gn = 0.0;
-    double gr[kNP] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if constexpr (KT >= 0) {
         using VG = VjpGroups<KT>;
         double acc[G];
@@ -363,21 +521,27 @@ __device__ __forceinline__ void vjp_finish_row_wave(const VjpArgs& a, const int 
         }
     }
     if (a.sel >= 0) {
-        const double S = selection_rt(a.sel, xm, a.S0, a.v_ref, a.alpha, a.v_min);
+        const double S = b4[1];
         if (S != 0.0 && Nm > 2.2e-308) {
-            const double h = brk_e[m] - lm;
+            const double h = b4[0] - lm;
             gn = fma(S, h, gn);
-            gr[3] += (a.log_scale ? a.S0 : 1.0) *
-                     (selection_shape(a.sel, xm, a.v_ref, a.alpha, a.v_min) * Nm * h);
-            if (a.sel == 2) gr[4] += S * log(xm / a.v_ref) * Nm * h;
+            gr[3] += b4[2] * Nm * h;
+            if (a.sel == 2) gr[4] += b4[3] * Nm * h;
         }
     }
     if (lane == 0) {
-        double* grow = a.gth_rows + (size_t)kNP * m;
 #pragma unroll
-        for (int p = 0; p < kNP; ++p)
-            if (gr[p] != 0.0) grow[p] += gr[p];
-        vjp_store_row(a, m, gn);
+        for (int p = 0; p < kNP; ++p) grow[p] = gr[p];
+        a.gN[m] = gn;
+        if (a.lam_acc) {
+            const double l = lacc + gn;
+            a.lam_acc[m] = l;
+            if (a.av_next) a.av_next[m] = fma(a.c2, gn, a.c1 * lbase);
+            if (a.lam_copy) {
+                a.lam_copy[m] = l;
+                a.av_start[m] = a.c_start * l;
+            }
+        }
     }
 }
 
@@ -392,19 +556,32 @@ __global__ __launch_bounds__(256)
 void vjp_part_kernel(VjpArgs a, double* part, int kc, int nch, double* brk_e)
 {
     __shared__ VjpSmem<KT> sm;
+    __shared__ VjpStage st;
     constexpr int G = VjpSmem<KT>::G;
     const int c = blockIdx.y;
     if (c >= nch) {
+        // breakage terms of row m: the gather and the selection values the
+        // finish pass needs (pow / log once here, beside the pair chunks)
         const int m = blockIdx.x * kWave + (c - nch) * kVjpBrkRows + (int)(threadIdx.x / kWave);
         if (m < a.n) {
             const double e = vjp_brk_gather_wave(a, m);
-            if ((threadIdx.x & (kWave - 1)) == 0) brk_e[m] = e;
+            if ((threadIdx.x & (kWave - 1)) == 0) {
+                const double xm = a.x[m];
+                const double S = selection_rt(a.sel, xm, a.S0, a.v_ref, a.alpha, a.v_min);
+                double* o = brk_e + 4 * (size_t)m;
+                o[0] = e;
+                o[1] = S;
+                o[2] = (a.log_scale ? a.S0 : 1.0) *
+                       selection_shape(a.sel, xm, a.v_ref, a.alpha, a.v_min);
+                o[3] = a.sel == 2 ? S * log(xm / a.v_ref) : 0.0;
+            }
         }
         return;
     }
     if constexpr (KT >= 0) {
         const int kb = c * kc, ke = min(a.n, kb + kc);
-        vjp_part_body<KT>(a, blockIdx.x * kWave, kb, ke, part + (size_t)c * G * a.n, sm);
+        vjp_part_body_staged<KT>(a, blockIdx.x * kWave, kb, ke, part + (size_t)c * G * a.n,
+                                 sm, st);
     }
 }
 
@@ -444,6 +621,8 @@ __device__ __forceinline__ VjpArgs instance_args(const VjpArgs& base, const Batc
     }
     a.n = d.n;
     a.coef = KernelCoef{d.b0, d.b_br, d.b_sh};
+    a.log_x0 = d.log_x0;
+    a.inv_log_r = d.inv_log_r;
     a.S0 = d.S0;
     a.v_ref = d.v_ref;
     a.alpha = d.alpha;
@@ -497,23 +676,25 @@ hipError_t enqueue_vjp_table(const double* x, double* tab, int n, hipStream_t s)
     return hipGetLastError();
 }
 
-// Column chunks of the single-instance VJP for n bins: enough work items to
-// fill the device (about 1024), at least 16 columns per chunk.
+// Column chunks of the single-instance VJP for n bins: about kVjpItems work
+// items, at least 16 and at most kVjpMaxCols columns per chunk.
+constexpr int kVjpItems = 512;
 static int vjp_chunks(int n)
 {
     const int nrb = (n + kWave - 1) / kWave;
-    int c = (1024 + nrb - 1) / nrb;
+    int c = (kVjpItems + nrb - 1) / nrb;
     c = std::min(c, 64);
     c = std::min(c, std::max(1, n / 16));
+    c = std::max(c, (n + kVjpMaxCols - 1) / kVjpMaxCols);
     return std::max(c, 1);
 }
 
 // Single instance: nch chunks of up to 2 groups per row, then the breakage
-// gather of every row.
+// terms of every row (4 doubles each).
 size_t vjp_part_doubles(int n, bool batched)
 {
     return batched ? (size_t)kBatchVjpChunks * 2 * (size_t)n
-                   : ((size_t)vjp_chunks(n) * 2 + 1) * (size_t)n;
+                   : ((size_t)vjp_chunks(n) * 2 + 4) * (size_t)n;
 }
 
 template <int KT>
@@ -572,6 +753,10 @@ hipError_t enqueue_vjp(const RhsProblem& p, const double* N, const double* lam,
     a.gN = gN;
     a.gth_rows = gth_rows;
     a.n = p.n;
+    if (p.agg) {
+        a.log_x0 = p.agg->log_x0;
+        a.inv_log_r = p.agg->inv_log_r;
+    }
     int kt = -1;
     if (p.batch) {
         const pbe_batch_plan_s* pl = p.batch;
