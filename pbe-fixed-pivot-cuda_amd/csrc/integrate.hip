@@ -22,15 +22,16 @@ __global__ void fill_zero_kernel(double* __restrict__ a, int n)
     if (i < n) a[i] = 0.0;
 }
 
-// mode 1: acc = k;          stage = N + (0.5 dt) k
+// mode 1: acc = k;          stage = N + (0.5 dt) k   (ysave = N if given)
 // mode 2: acc = acc + 2 k;  stage = N + (0.5 dt) k
 // mode 3: acc = acc + 2 k;  stage = N + dt k
-// mode 4: N = N + (dt/6)(acc + k)
-// mode 5 (Euler): N = N + dt k
-// k is zeroed after it is read.
-__global__ void rk_stage_kernel(double* __restrict__ N, double* __restrict__ k,
+// mode 4: Nout = N + (dt/6)(acc + k)
+// mode 5 (Euler): Nout = N + dt k
+// N is read and Nout written (the same array for a step in place); k is
+// zeroed after it is read.
+__global__ void rk_stage_kernel(const double* N, double* Nout, double* __restrict__ k,
                                 double* __restrict__ acc, double* __restrict__ stage,
-                                int n, double dt, int mode)
+                                double* __restrict__ ysave, int n, double dt, int mode)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -40,6 +41,7 @@ __global__ void rk_stage_kernel(double* __restrict__ N, double* __restrict__ k,
     case 1:
         acc[i] = ki;
         stage[i] = __dadd_rn(N[i], __dmul_rn(__dmul_rn(0.5, dt), ki));
+        if (ysave) ysave[i] = N[i];
         break;
     case 2:
         acc[i] = __dadd_rn(acc[i], __dmul_rn(2.0, ki));
@@ -50,10 +52,10 @@ __global__ void rk_stage_kernel(double* __restrict__ N, double* __restrict__ k,
         stage[i] = __dadd_rn(N[i], __dmul_rn(dt, ki));
         break;
     case 4:
-        N[i] = __dadd_rn(N[i], __dmul_rn(__ddiv_rn(dt, 6.0), __dadd_rn(acc[i], ki)));
+        Nout[i] = __dadd_rn(N[i], __dmul_rn(__ddiv_rn(dt, 6.0), __dadd_rn(acc[i], ki)));
         break;
     default:
-        N[i] = __dadd_rn(N[i], __dmul_rn(dt, ki));
+        Nout[i] = __dadd_rn(N[i], __dmul_rn(dt, ki));
         break;
     }
 }
@@ -87,8 +89,9 @@ hipError_t enqueue_rhs(const RhsProblem& p, const double* Nin, double* rhs, hipS
     return hipGetLastError();
 }
 
-hipError_t enqueue_rk4_step_stages(const RhsProblem& p, double* N, double dt, double* work,
-                                   double* const* stages, hipStream_t s)
+hipError_t enqueue_rk4_step_io(const RhsProblem& p, const double* Nin, double* Nout, double dt,
+                               double* work, double* const* stages, double* ysave,
+                               hipStream_t s)
 {
     const int n = p.n;
     double* k = work;
@@ -97,16 +100,23 @@ hipError_t enqueue_rk4_step_stages(const RhsProblem& p, double* N, double dt, do
     double* st2 = stages ? stages[1] : st1;
     double* st3 = stages ? stages[2] : st1;
     const dim3 g(blocks_for(n)), b(256);
+    double* const none = nullptr;
     hipError_t e;
-    if ((e = enqueue_rhs(p, N, k, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(rk_stage_kernel, g, b, 0, s, N, k, acc, st1, n, dt, 1);
+    if ((e = enqueue_rhs(p, Nin, k, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(rk_stage_kernel, g, b, 0, s, Nin, Nout, k, acc, st1, ysave, n, dt, 1);
     if ((e = enqueue_rhs(p, st1, k, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(rk_stage_kernel, g, b, 0, s, N, k, acc, st2, n, dt, 2);
+    hipLaunchKernelGGL(rk_stage_kernel, g, b, 0, s, Nin, Nout, k, acc, st2, none, n, dt, 2);
     if ((e = enqueue_rhs(p, st2, k, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(rk_stage_kernel, g, b, 0, s, N, k, acc, st3, n, dt, 3);
+    hipLaunchKernelGGL(rk_stage_kernel, g, b, 0, s, Nin, Nout, k, acc, st3, none, n, dt, 3);
     if ((e = enqueue_rhs(p, st3, k, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(rk_stage_kernel, g, b, 0, s, N, k, acc, st1, n, dt, 4);
+    hipLaunchKernelGGL(rk_stage_kernel, g, b, 0, s, Nin, Nout, k, acc, st1, none, n, dt, 4);
     return hipGetLastError();
+}
+
+hipError_t enqueue_rk4_step_stages(const RhsProblem& p, double* N, double dt, double* work,
+                                   double* const* stages, hipStream_t s)
+{
+    return enqueue_rk4_step_io(p, N, N, dt, work, stages, nullptr, s);
 }
 
 hipError_t enqueue_rk4_step(const RhsProblem& p, double* N, double dt, double* work,
@@ -120,8 +130,9 @@ hipError_t enqueue_euler_step(const RhsProblem& p, double* N, double dt, double*
 {
     hipError_t e;
     if ((e = enqueue_rhs(p, N, work, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(rk_stage_kernel, dim3(blocks_for(p.n)), dim3(256), 0, s, N, work,
-                       (double*)nullptr, (double*)nullptr, p.n, dt, 5);
+    double* const none = nullptr;
+    hipLaunchKernelGGL(rk_stage_kernel, dim3(blocks_for(p.n)), dim3(256), 0, s, N, N, work, none,
+                       none, none, p.n, dt, 5);
     return hipGetLastError();
 }
 

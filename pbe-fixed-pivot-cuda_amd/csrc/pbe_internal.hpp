@@ -104,6 +104,11 @@ hipError_t enqueue_rk4_step(const RhsProblem& p, double* N, double dt, double* w
 // a scratch slot of work).
 hipError_t enqueue_rk4_step_stages(const RhsProblem& p, double* N, double dt, double* work,
                                    double* const* stages, hipStream_t s);
+// As above from Nin to Nout (Nin is left as it is unless it is Nout), and
+// Nin also copied to ysave (when non-null) by the first stage kernel.
+hipError_t enqueue_rk4_step_io(const RhsProblem& p, const double* Nin, double* Nout, double dt,
+                               double* work, double* const* stages, double* ysave,
+                               hipStream_t s);
 // One explicit Euler step; work = n doubles, zero on entry and exit.
 hipError_t enqueue_euler_step(const RhsProblem& p, double* N, double dt, double* work,
                               hipStream_t s);
@@ -134,7 +139,9 @@ size_t vjp_part_doubles(int n, bool batched);
 struct AdjointLayout {
     bool store_all;
     int nseg;
-    size_t ckpt, seg, tab, lam, lam2, av, mu, rk, gth, part, total;
+    // seg2, rk2, mu2: the second segment buffer and the integrator scratch of
+    // the recompute running beside the reverse sweep (checkpoint mode only)
+    size_t ckpt, seg, seg2, tab, lam, lam2, av, mu, rk, rk2, mu2, gth, part, total;
 };
 AdjointLayout adjoint_layout(int n, int steps, int every);
 hipError_t enqueue_adjoint_forward(const RhsProblem& p, double* N, double dt, int steps,

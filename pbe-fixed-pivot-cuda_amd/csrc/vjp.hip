@@ -20,6 +20,7 @@
 // array (each element written by one lane) and reduced in a fixed order at
 // the end.
 #include <algorithm>
+#include <mutex>
 
 #include "batch_plan.hpp"
 #include "pbe_device.hpp"
@@ -869,12 +870,15 @@ AdjointLayout adjoint_layout(int n, int steps, int every)
     size_t off = 0;
     L.ckpt = off;  off += L.store_all ? 0 : nn * (size_t)L.nseg;
     L.seg = off;   off += nn * 4 * seg_steps;
+    L.seg2 = off;  off += L.store_all ? 0 : nn * 4 * seg_steps;
     L.tab = off;   off += 4 * nn;
     L.lam = off;   off += nn;
     L.lam2 = off;  off += nn;
     L.av = off;    off += 2 * nn;
     L.mu = off;    off += nn;
     L.rk = off;    off += 3 * nn;
+    L.rk2 = off;   off += L.store_all ? 0 : 3 * nn;
+    L.mu2 = off;   off += L.store_all ? 0 : nn;
     L.gth = off;   off += (size_t)kNP * nn;
     L.part = off;  off += std::max(vjp_part_doubles(n, false), vjp_part_doubles(n, true));
     L.total = off; // doubles
@@ -890,16 +894,40 @@ hipError_t enqueue_adjoint_forward(const RhsProblem& p, double* N, double dt, in
     hipError_t e = enqueue_fill_zero(rk, p.n, s);
     for (int i = 0; e == hipSuccess && i < steps; ++i) {
         if (L.store_all) {
+            // the first stage kernel keeps Y1 = N in Yi
             double* Yi = ws + L.seg + nn * 4 * (size_t)i;
             double* st[3] = {Yi + nn, Yi + 2 * nn, Yi + 3 * nn};
-            e = enqueue_copy(N, Yi, p.n, s);
-            if (e == hipSuccess) e = enqueue_rk4_step_stages(p, N, dt, rk, st, s);
+            e = enqueue_rk4_step_io(p, N, N, dt, rk, st, Yi, s);
             continue;
         }
         if (i % every == 0) e = enqueue_copy(N, ws + L.ckpt + nn * (i / every), p.n, s);
         if (e == hipSuccess) e = enqueue_rk4_step_stages(p, N, dt, rk, nullptr, s);
     }
     return e;
+}
+
+// The library's second stream on the current device (default priority,
+// created on first use and kept): the checkpointed adjoint recomputes the
+// next segment there while the reverse sweep of the current one runs on the
+// caller's stream.
+static hipError_t side_stream(hipStream_t* out)
+{
+    static std::mutex mu;
+    static hipStream_t streams[64] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!streams[dev]) {
+        e = hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            streams[dev] = nullptr;
+            return e;
+        }
+    }
+    *out = streams[dev];
+    return hipSuccess;
 }
 
 hipError_t enqueue_adjoint_backward(const RhsProblem& p, const double* gN_final, double* gN0,
@@ -915,60 +943,98 @@ hipError_t enqueue_adjoint_backward(const RhsProblem& p, const double* gN_final,
     double* av0 = ws + L.av;
     double* av1 = ws + L.av + nn;
     double* mu = ws + L.mu;
-    double* rk = ws + L.rk;
     double* gth = ws + L.gth;
-    double* seg = ws + L.seg;
     double* part = ws + L.part;
     hipError_t e;
     if ((e = enqueue_vjp_table(p.x, tab, n, s)) != hipSuccess) return e;
     if ((e = enqueue_fill_zero(gth, kNP * n, s)) != hipSuccess) return e;
-    if ((e = enqueue_fill_zero(rk, n, s)) != hipSuccess) return e;
     if ((e = enqueue_copy(gN_final, lam, n, s)) != hipSuccess) return e;
     const dim3 g = grid1(n), b(256);
-    for (int sg = L.nseg - 1; sg >= 0; --sg) {
-        const int s0 = L.store_all ? 0 : sg * every;
-        const int s1 = L.store_all ? steps : min(steps, s0 + every);
-        double* Y = seg;   // Y[(i - s0)*4 + q]
-        // recompute the segment, keeping every stage state
-        if (!L.store_all && (e = enqueue_copy(ws + L.ckpt + nn * sg, Y, n, s)) != hipSuccess)
-            return e;
-        for (int i = s0; i < s1 && !L.store_all; ++i) {
+
+    // Recompute segment sg from its checkpoint into Y (every stage state),
+    // on stream ss with integrator scratch rkw (k zero on entry and exit)
+    // and muw.
+    auto recompute = [&](int sg, double* Y, double* rkw, double* muw,
+                         hipStream_t ss) -> hipError_t {
+        const int s0 = sg * every, s1 = min(steps, s0 + every);
+        hipError_t er = enqueue_copy(ws + L.ckpt + nn * sg, Y, n, ss);
+        for (int i = s0; er == hipSuccess && i < s1; ++i) {
             double* Yi = Y + nn * 4 * (size_t)(i - s0);
             double* st[3] = {Yi + nn, Yi + 2 * nn, Yi + 3 * nn};
-            // the step advances a copy: Y1 of step i+1 follows Y4 of step i
-            double* next = (i + 1 < s1) ? Yi + 4 * nn : nullptr;
-            if (next) {
-                if ((e = enqueue_copy(Yi, next, n, s)) != hipSuccess) return e;
-                if ((e = enqueue_rk4_step_stages(p, next, dt, rk, st, s)) != hipSuccess) return e;
-            } else {
-                // last step of the segment: only the stage states are needed
-                if ((e = enqueue_copy(Yi, mu, n, s)) != hipSuccess) return e;
-                if ((e = enqueue_rk4_step_stages(p, mu, dt, rk, st, s)) != hipSuccess) return e;
-            }
+            // the step goes from Yi to Y1 of step i+1; the segment's last
+            // step only needs its stage states
+            double* next = (i + 1 < s1) ? Yi + 4 * nn : muw;
+            er = enqueue_rk4_step_io(p, Yi, next, dt, rkw, st, nullptr, ss);
         }
-        // reverse sweep over the segment; each VJP's finish pass also adds
-        // mu_s to the new lam and forms the next stage's cotangent, and the
-        // last stage's finish starts the next step (lam2 <- lam, av0 <- dt/6 lam
-        // after the swap), so only the first step needs adj_start_kernel
+        return er;
+    };
+    // Reverse sweep over steps [s0, s1) with stage states Y; each VJP's
+    // finish pass also adds mu_s to the new lam and forms the next stage's
+    // cotangent, and the last stage's finish starts the next step (lam2 <-
+    // lam, av0 <- dt/6 lam after the swap), so only the first step needs
+    // adj_start_kernel.
+    auto reverse = [&](int s0, int s1, double* Y) -> hipError_t {
+        hipError_t er;
         for (int i = s1 - 1; i >= s0; --i) {
             double* Yi = Y + nn * 4 * (size_t)(i - s0);
             if (i == steps - 1)
                 hipLaunchKernelGGL(adj_start_kernel, g, b, 0, s, lam, lam2, av0, n, dt / 6.0);
             const AdjointStageFuse f4{lam2, av1, lam, dt / 3.0, dt};
-            if ((e = enqueue_vjp(p, Yi + 3 * nn, av0, tab, mu, gth, part, s, &f4)) != hipSuccess)
-                return e;
+            if ((er = enqueue_vjp(p, Yi + 3 * nn, av0, tab, mu, gth, part, s, &f4)) != hipSuccess)
+                return er;
             const AdjointStageFuse f3{lam2, av0, lam, dt / 3.0, 0.5 * dt};
-            if ((e = enqueue_vjp(p, Yi + 2 * nn, av1, tab, mu, gth, part, s, &f3)) != hipSuccess)
-                return e;
+            if ((er = enqueue_vjp(p, Yi + 2 * nn, av1, tab, mu, gth, part, s, &f3)) != hipSuccess)
+                return er;
             const AdjointStageFuse f2{lam2, av1, lam, dt / 6.0, 0.5 * dt};
-            if ((e = enqueue_vjp(p, Yi + nn, av0, tab, mu, gth, part, s, &f2)) != hipSuccess)
-                return e;
+            if ((er = enqueue_vjp(p, Yi + nn, av0, tab, mu, gth, part, s, &f2)) != hipSuccess)
+                return er;
             const AdjointStageFuse f1{lam2, nullptr, nullptr, 0.0, 0.0, lam, av0, dt / 6.0};
-            if ((e = enqueue_vjp(p, Yi, av1, tab, mu, gth, part, s, &f1)) != hipSuccess) return e;
+            if ((er = enqueue_vjp(p, Yi, av1, tab, mu, gth, part, s, &f1)) != hipSuccess)
+                return er;
             double* t = lam;
             lam = lam2;
             lam2 = t;
         }
+        return hipSuccess;
+    };
+
+    if (L.store_all) {
+        if ((e = reverse(0, steps, ws + L.seg)) != hipSuccess) return e;
+    } else {
+        // Segment sg's states go to buffer sg % 2.  The recompute of segment
+        // sg - 1 runs on the side stream while the reverse sweep of segment
+        // sg runs on s: the reverse sweep of sg waits for its recompute
+        // (ev_rec), and the recompute into a buffer waits for the reverse
+        // sweep that last read it (ev_rev, recorded after the sweep of
+        // sg + 1).  An event's later record does not move an earlier wait.
+        double* buf[2] = {ws + L.seg, ws + L.seg2};
+        double* rk2 = ws + L.rk2;
+        double* mu2 = ws + L.mu2;
+        hipStream_t s2 = nullptr;
+        if ((e = side_stream(&s2)) != hipSuccess) return e;
+        hipEvent_t ev[3] = {nullptr, nullptr, nullptr};   // start, rec, rev
+        for (int k = 0; k < 3 && e == hipSuccess; ++k)
+            e = hipEventCreateWithFlags(&ev[k], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventRecord(ev[0], s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s2, ev[0], 0);
+        if (e == hipSuccess) e = enqueue_fill_zero(rk2, n, s2);
+        const int last = L.nseg - 1;
+        if (e == hipSuccess) e = recompute(last, buf[last % 2], rk2, mu2, s2);
+        if (e == hipSuccess) e = hipEventRecord(ev[1], s2);
+        for (int sg = last; e == hipSuccess && sg >= 0; --sg) {
+            e = hipStreamWaitEvent(s, ev[1], 0);
+            if (e == hipSuccess && sg > 0) {
+                if (sg < last) e = hipStreamWaitEvent(s2, ev[2], 0);
+                if (e == hipSuccess) e = recompute(sg - 1, buf[(sg - 1) % 2], rk2, mu2, s2);
+                if (e == hipSuccess) e = hipEventRecord(ev[1], s2);
+            }
+            const int s0 = sg * every, s1 = min(steps, s0 + every);
+            if (e == hipSuccess) e = reverse(s0, s1, buf[sg % 2]);
+            if (e == hipSuccess) e = hipEventRecord(ev[2], s);
+        }
+        for (int k = 0; k < 3; ++k)
+            if (ev[k]) (void)hipEventDestroy(ev[k]);
+        if (e != hipSuccess) return e;
     }
     if ((e = enqueue_copy(lam, gN0, n, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(gth_reduce_kernel, dim3(1), dim3(256), 0, s, gth, gtheta, n);
