@@ -8,9 +8,12 @@ seeded lognormal N already resident in HBM.  Work units per step:
 n(n+1)/2 pairs + n*n_quad breakage updates (dense: pairs with N = 0 count).
 
 Prints one JSON line.  ``roofline`` refers to the dominant work, the
-aggregation call (at n = 16384 two launches: the diagonal-band sweep and the
-column strips): achieved = 21 FP64 FLOP per pair x pairs / average duration
-of the call, measured with HIP events on the launch stream around it.
+aggregation call (at n = 16384 one fused launch of the diagonal-band sweep
+items and the column strips): achieved = 21 FP64 FLOP per pair x pairs /
+average duration of the call, measured with HIP events on the launch stream
+around it; beside it the FP64 FLOPs the call executes (rocprofv3
+SQ_INSTS_VALU_FLOPS_FP64 of the same launch, committed under profiles/) over
+the same time.
 ``cpu_baseline`` times the OpenMP FP64 restatement of the reference
 (oracle/pbe_cpu_port.c) on this host on a bounded sample of the same work.
 """
@@ -41,6 +44,8 @@ def parse():
                     help="wall-clock budget of the CPU baseline sample")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the batched and adjoint comparisons")
+    ap.add_argument("--no-fit", action="store_true",
+                    help="skip the batched joint fit (its thousands of launches swamp a trace)")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the n = 1024 / 4096 / GL16 / 65536 variants (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -423,11 +428,13 @@ def main():
     # (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-kernel calibrated); null when
     # they were taken on other launches
     traffic = None
+    executed = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             tj = json.load(f)
         if tj.get("launch") == agg_label:
             traffic = tj.get("hbm_bytes_per_launch")
+            executed = tj.get("fp64_flop_executed_per_launch")
 
     out = {
         "metric": METRIC,
@@ -462,6 +469,10 @@ def main():
             "traffic": traffic,
             "kernel": agg_label,
             "flop_per_launch": flops,
+            "executed_fp64_flop_per_launch": executed,
+            "executed_tflops": (executed / (agg_ms * 1e-3) / 1e12) if executed else None,
+            "executed_frac": (executed / (agg_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS)
+            if executed else None,
             "avg_launch_ms": agg_ms,
             "median_launch_ms": agg_median,
         },
@@ -483,7 +494,8 @@ def main():
         out["forms"]["deterministic"]["over_atomic"] = (
             out["forms"]["deterministic"]["ms_per_step"] / ms_step)
         out["forms"]["flocculation_two_stage"] = flocculation_form()
-        out["forms"]["batched_fit"] = batched_fit_form(hip, P)
+        if not args.no_fit:
+            out["forms"]["batched_fit"] = batched_fit_form(hip, P)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, x, agg, brk, t_q, bw_q, args.cpu_seconds)
     print(json.dumps(out))
