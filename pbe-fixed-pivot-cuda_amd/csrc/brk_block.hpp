@@ -21,10 +21,13 @@ struct BrkArgs {
 constexpr int kBrkParents = 256;   // parents per workgroup
 constexpr int kBrkWin = 2048;      // bins in the LDS window (16 KB)
 
+constexpr int kBrkQuadLds = 64;   // quadrature points held in LDS (more: read from global)
+
 struct BrkSmem {
     double sn[kBrkParents];
     double win[kBrkWin];
     double xw[kBrkWin];
+    double tq[kBrkQuadLds], bw[kBrkQuadLds];
 };
 
 // Death and birth of parents [p0, p0 + 256) of the instance described by `a`.
@@ -55,6 +58,14 @@ __device__ __forceinline__ void brk_block_body(const BrkArgs& a, const int p0, B
         const int b = wbase + i;
         xw[i] = (b >= 0 && b < n) ? a.x[b] : 0.0;
     }
+    // the daughter table and the grid's end pivots come with the same round
+    // trip as the window
+    if (tid < nq && tid < kBrkQuadLds) {
+        sm.tq[tid] = a.t_q[tid];
+        sm.bw[tid] = a.bw_q[tid];
+    }
+    const double xfirst = a.x[0];
+    const double xlast = a.x[n - 1];
     __syncthreads();
 
     auto put = [&](int i, double c) {
@@ -82,8 +93,6 @@ __device__ __forceinline__ void brk_block_body(const BrkArgs& a, const int p0, B
     }
     __syncthreads();
 
-    const double xfirst = a.x[0];
-    const double xlast = a.x[n - 1];
     const double xwlo = xw[wlo - wbase];
     const double xwhi = xw[whi - wbase];
     const int items = P * nq;
@@ -94,8 +103,10 @@ __device__ __forceinline__ void brk_block_body(const BrkArgs& a, const int p0, B
         if (j >= n) continue;
         const double s = sn[i];
         if (s == 0.0) continue;
-        const double c = s * a.bw_q[q];
-        const double v = xw[j - wbase] * a.t_q[q];
+        const double bwq = q < kBrkQuadLds ? lds_ld(&sm.bw[q]) : glb_ld(&a.bw_q[q]);
+        const double tqq = q < kBrkQuadLds ? lds_ld(&sm.tq[q]) : glb_ld(&a.t_q[q]);
+        const double c = s * bwq;
+        const double v = xw[j - wbase] * tqq;
         if (v <= xfirst) {
             put(0, c);
         } else if (v >= xlast) {

@@ -308,12 +308,23 @@ __device__ __forceinline__ void vjp_part_body_staged(const VjpArgs& a, const int
         auto iat = [&](int i) {
             return i - kb < nw ? lds_ld(&st.wid[i - kb]) : glb_ld(&a.tab[4 * i + 2]);
         };
-        // bracket of the first pair of this quarter (cur >= wb: v >= x_wb)
+        // bracket of the first pair of this quarter: v0 >= x_lo with
+        // lo = max(m, wb), and for a far pair (the row far below the
+        // column, or the column far below the row) the bracket is (lo,
+        // lo + 1), one compare; else a search
         int cur = -1;
         double xc = 0.0, xc1 = 0.0, lc = 0.0, lc1 = 0.0, idc = 0.0;
         {
             const double v0 = xm + st.wx[wb - kb];
-            if (v0 < xlast) {
+            const int lo0 = max(m, wb);
+            if (v0 < xlast && v0 < xat(lo0 + 1)) {
+                cur = lo0;
+                xc = xat(cur);
+                xc1 = xat(cur + 1);
+                lc = lat(cur);
+                lc1 = lat(cur + 1);
+                idc = iat(cur);
+            } else if (v0 < xlast) {
                 int hi;
                 if (v0 < st.wx[nw - 1]) {
                     int lo = wb - kb, h = nw - 1;   // wx[lo] <= v0 < wx[h]
