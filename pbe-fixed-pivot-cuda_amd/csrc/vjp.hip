@@ -177,17 +177,22 @@ __device__ __forceinline__ void vjp_part_body(const VjpArgs& a, const int m0, co
         const double xm = a.x[m], lm = a.lam[m];
         const double cm = a.tab[4 * m + 0], icm = a.tab[4 * m + 1];
         const double xlast = a.x[n - 1], llast = a.lam[n - 1];
-        // bracket of the first pair of this quarter: on a geometric grid the
-        // log-ratio guess with its exact fix-up (a few loads), else a binary
-        // search
+        // bracket of the first pair of this quarter: for a far pair one
+        // compare; else on a geometric grid the log-ratio guess with its exact
+        // fix-up (a few loads), or a binary search
         int cur = -1;
         double xc = 0.0, xc1 = 0.0, lc = 0.0, lc1 = 0.0, idc = 0.0;
         {
+            // v0 >= x_lo with lo = max(m, wb): a far pair's bracket is
+            // (lo, lo + 1), one compare
             const double v0 = xm + a.x[wb];
+            const int lo0 = max(m, wb);
             if (v0 < xlast) {
-                cur = (a.inv_log_r != 0.0
+                cur = v0 < a.x[lo0 + 1]
+                    ? lo0
+                    : (a.inv_log_r != 0.0
                            ? bracket_geometric(a.x, n, v0, a.log_x0, a.inv_log_r)
-                           : bracket_search(a.x, max(m, wb), n - 1, v0)) - 1;
+                           : bracket_search(a.x, lo0, n - 1, v0)) - 1;
                 xc = a.x[cur];
                 xc1 = a.x[cur + 1];
                 lc = a.lam[cur];
