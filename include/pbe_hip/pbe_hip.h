@@ -93,16 +93,19 @@ hipError_t pbe_launch_aggregation_rhs(const double* N, const double* x,
  * 30-33).  tile_u = columns per lane: column blocks and diagonal-band tiles
  * are 64*tile_u wide; band_split = row slices per diagonal-band tile (each
  * slice one workgroup; offset segments per block for the 256-wide band
- * sweeps); form = 1: the band and the column strips as two launches, 2: one
+ * sweeps); form = 1: the band and the column strips as two launches (one
+ * launch of every tile when the strips would hold too little work), 2: one
  * launch running both side by side (taken for tile_u = 4 up to 64 column
- * blocks, otherwise the same as 1).  0 picks the library's autotuned choice
+ * blocks, otherwise the same as 1), 3: one launch of every tile (the
+ * diagonal tiles within the near-pair reach as row slices).  0 picks the
+ * library's autotuned choice
  * for (n, kernel_type, grid reach), where the reach of a geometric grid
  * (inv_log_r > 0) is log_r(1/(r-1)) bins: how far below the diagonal a pair
  * can still land beyond its bracket (k, k+1). */
 typedef struct pbe_agg_options {
     int tile_u;       /* 0, 1, 2, 4 or 8 */
     int band_split;   /* 0, 1, 2, 4, 8 or 16 */
-    int form;         /* 0, 1 or 2 */
+    int form;         /* 0, 1, 2 or 3 */
 } pbe_agg_options;
 
 /* pbe_launch_aggregation_rhs with explicit launch options (same results). */

@@ -8,9 +8,10 @@ Single-instance launches (csrc/tile_table.inc, rows {n_max, kernel_type,
 reach_max, U, S, F}): for every collision kernel, grid-reach class and size
 bucket, every block width 64*U (U = 1, 2, 4, 8 columns per lane) and band
 split S = 1, 2, 4, 8, 16 (row slices per diagonal-band tile; offset segments
-per block for the 256-wide band sweeps) is timed, at U = 4 both as two
-launches (F = 1) and as the one fused launch (F = 2), and the fastest
-(U, S, F) is written.
+per block for the 256-wide band sweeps) is timed as two launches (band,
+then strips; F = 1) and as one launch of every tile (F = 3), at U = 4 also
+as the one fused band + strip launch (F = 2), and the fastest (U, S, F) is
+written.
 
 Batched launches (csrc/batch_tile_table.inc, rows {n_max, kernel_type,
 reach_max, U}): for every kernel, reach class and instance-size bucket, a
@@ -27,10 +28,16 @@ classes are reach <= 64, 128, 256, 512, 1200 and beyond (tuned at r = 1.06,
 n ln r <= 330, so that every pivot and every kernel value stays a normal
 double; larger n reuses the largest tuned size's row.
 
-Workload: x from 1e-18, N_i = 1e6 r^(-i/2) (1 + 0.05 u_i) -- nonzero and
-normal on the whole grid, so every window flush and atomic is taken -- and
-the kernel coefficients of the flocculation case.  Each shape is timed over
-a hipGraph of back-to-back launches (so small sizes measure the GPU, not the
+Workload: x from 1e-18 and the kernel coefficients of the flocculation
+case.  Single launches are timed on the population of the single-instance
+throughput workload: lognormal, median x_{n/4}, sigma_ln 0.8, +-5 % noise,
+whose tail underflows to zero some 40 sigma out, so that the flushes of
+empty bins drop out as they do in practice (the launch forms differ in how
+many flushes they make: on a population nonzero everywhere the choice
+between near-equal forms can flip).  Batched launches are timed on
+N_i = 1e6 r^(-i/2) (1 + 0.05 u_i), nonzero and normal on the whole grid,
+so every window flush and atomic is taken.  Each shape is timed over a
+hipGraph of back-to-back launches (so small sizes measure the GPU, not the
 host's launch rate), best of three replays.
 
 Both tables are written exactly as generated; the library holds no tuning
@@ -67,6 +74,12 @@ def grid(n, r, general, seed):
     if general:
         return P.jittered_grid(n, 1e-18, r, 0.3, seed)
     return P.geometric_grid(n, 1e-18, r)
+
+
+def lognormal_populations(x, seed):
+    from pbe_hip import problems as P
+    n = len(x)
+    return P.lognormal_ic(x, x[n // 4], 0.8, 1e12, noise=0.05, seed=seed)
 
 
 def populations(n, r, seed):
@@ -137,8 +150,8 @@ def tune_single(pbe, hip, timer, sizes, kernels, log):
         general = reach_max < 0
         for n in class_sizes(sizes, r):
             x = grid(n, r, general, seed=n)
-            N = populations(n, r, seed=n + 1)
-            dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
+            dN = hip.DeviceArray.from_host(lognormal_populations(x, seed=n + 2))
+            dx = hip.DeviceArray.from_host(x)
             drhs = hip.DeviceArray(n)
             drhs.zero()
             line = []
@@ -155,7 +168,7 @@ def tune_single(pbe, hip, timer, sizes, kernels, log):
                                   f"n {n} kernel {k} tile {u} split {w} form {f}")
                     return timer(launch)
                 t = {(u, w, f): run(u, w, f) for u in TILES for w in SPLITS
-                     for f in ((1, 2) if u == 4 else (1,))}
+                     for f in ((1, 2, 3) if u == 4 else (1, 3))}
                 bu, bw, bf = min(t, key=t.get)
                 best.setdefault((k, reach_max), []).append((n, t))
                 line.append(f"k{k}:U{bu}S{bw}F{bf} {t[(bu, bw, bf)] * 1e3:.1f}us")
@@ -303,8 +316,8 @@ def main():
                      "MI355X (gfx950); do not edit."],
                     merged_rows(best))
         summary += ["Single launches: chosen U (columns per lane), S (band split), F "
-                    "(launch form: 1 two launches, 2 fused) and the time of one "
-                    "aggregation call in us.", ""]
+                    "(launch form: 1 two launches, 2 fused band + strips, 3 one tile launch) "
+                    "and the time of one aggregation call in us.", ""]
         summary += summary_lines(best, batch=False)
     if args.part in ("batch", "both"):
         bbest = tune_batch(pbe, hip, timer, bsizes, kernels, log)

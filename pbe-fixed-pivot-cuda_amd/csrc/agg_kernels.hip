@@ -298,7 +298,8 @@ static StripPlan make_strip_plan(int n, int U, bool band, double inv_log_r)
 
 // Which launch(es) an aggregation call makes at block width U (the tuned
 // band split and form given): the fused band + strip launch, one launch of
-// every tile, or a band launch followed by the strip launch.
+// every tile, or a band launch followed by the strip launch.  Form 3 asks
+// for the tile launch whatever the strips' share of the work.
 enum class AggForm { Fused, Tiles, BandStrips };
 struct AggLaunch {
     AggForm form;
@@ -321,14 +322,14 @@ static int tile_band_diags(int T, double inv_log_r)
 }
 
 template <int KT, int U>
-static AggLaunch plan_agg_t(int n, double inv_log_r, int split, bool fused)
+static AggLaunch plan_agg_t(int n, double inv_log_r, int split, int form)
 {
     constexpr int T = AggShape<U>::T;
     const int nb = (n + T - 1) / T;
     AggLaunch L;
     L.sep = make_strip_plan(n, U, false, inv_log_r);
     L.nseg = split;
-    if (fused && T == kBandT && nb <= kFusedMaxBlocks && L.sep.total > 0) {
+    if (form == 2 && T == kBandT && nb <= kFusedMaxBlocks && L.sep.total > 0) {
         // mid sizes: sweep items on at most half of the resident slots of
         // the fused kernel, strip shares on the rest (empty shares return)
         const long long Gf = fused_resident_groups<KT>();
@@ -343,9 +344,10 @@ static AggLaunch plan_agg_t(int n, double inv_log_r, int split, bool fused)
             return L;
         }
     }
-    L.groups = resident_groups<KT, U>();
-    if (L.sep.total < L.groups) {
-        // too little far work for the strips: one launch of all tiles
+    L.groups = form == 3 ? 0 : resident_groups<KT, U>();
+    if (form == 3 || L.sep.total < L.groups) {
+        // asked for, or too little far work for the strips: one launch of
+        // all tiles
         const long long tiles = (long long)nb * (nb + 1) / 2;
         L.form = AggForm::Tiles;
         L.diags = tile_band_diags(T, inv_log_r);
@@ -359,11 +361,11 @@ static AggLaunch plan_agg_t(int n, double inv_log_r, int split, bool fused)
 }
 
 template <int KT, int U>
-static hipError_t launch_agg_t(AggArgs args, int split, bool fused, hipStream_t stream)
+static hipError_t launch_agg_t(AggArgs args, int split, int form, hipStream_t stream)
 {
     constexpr int T = AggShape<U>::T;
     args.nb = (args.n + T - 1) / T;
-    const AggLaunch L = plan_agg_t<KT, U>(args.n, args.inv_log_r, split, fused);
+    const AggLaunch L = plan_agg_t<KT, U>(args.n, args.inv_log_r, split, form);
     if (L.items > 0x7fffffffLL) return hipErrorInvalidValue;
     switch (L.form) {
     case AggForm::Fused:
@@ -392,16 +394,16 @@ static hipError_t launch_agg_t(AggArgs args, int split, bool fused, hipStream_t 
 }
 
 template <int U>
-static hipError_t launch_agg_u(const AggArgs& args, int kernel, int split, bool fused,
+static hipError_t launch_agg_u(const AggArgs& args, int kernel, int split, int form,
                                hipStream_t stream)
 {
     switch (kernel) {
-    case 0: return launch_agg_t<0, U>(args, split, fused, stream);
-    case 1: return launch_agg_t<1, U>(args, split, fused, stream);
-    case 2: return launch_agg_t<2, U>(args, split, fused, stream);
-    case 3: return launch_agg_t<3, U>(args, split, fused, stream);
-    case 4: return launch_agg_t<4, U>(args, split, fused, stream);
-    case 5: return launch_agg_t<5, U>(args, split, fused, stream);
+    case 0: return launch_agg_t<0, U>(args, split, form, stream);
+    case 1: return launch_agg_t<1, U>(args, split, form, stream);
+    case 2: return launch_agg_t<2, U>(args, split, form, stream);
+    case 3: return launch_agg_t<3, U>(args, split, form, stream);
+    case 4: return launch_agg_t<4, U>(args, split, form, stream);
+    case 5: return launch_agg_t<5, U>(args, split, form, stream);
     default: return hipErrorInvalidValue;
     }
 }
@@ -420,26 +422,25 @@ hipError_t launch_aggregation(const double* N, const double* x, double* rhs,
     args.log_x0 = p.log_x0;
     args.inv_log_r = p.inv_log_r;
     const int S = band_split_or_default(band_split);
-    const bool fused = form == 2;
     switch (tile_u) {
-    case 1: return launch_agg_u<1>(args, p.kernel_type, S, fused, stream);
-    case 2: return launch_agg_u<2>(args, p.kernel_type, S, fused, stream);
-    case 8: return launch_agg_u<8>(args, p.kernel_type, S, fused, stream);
-    default: return launch_agg_u<4>(args, p.kernel_type, S, fused, stream);
+    case 1: return launch_agg_u<1>(args, p.kernel_type, S, form, stream);
+    case 2: return launch_agg_u<2>(args, p.kernel_type, S, form, stream);
+    case 8: return launch_agg_u<8>(args, p.kernel_type, S, form, stream);
+    default: return launch_agg_u<4>(args, p.kernel_type, S, form, stream);
     }
 }
 
 template <int U>
-static TilesShape tiles_shape_u(int n, int kernel, double inv_log_r, int split, bool fused)
+static TilesShape tiles_shape_u(int n, int kernel, double inv_log_r, int split, int form)
 {
     AggLaunch L;
     switch (kernel) {
-    case 0: L = plan_agg_t<0, U>(n, inv_log_r, split, fused); break;
-    case 1: L = plan_agg_t<1, U>(n, inv_log_r, split, fused); break;
-    case 2: L = plan_agg_t<2, U>(n, inv_log_r, split, fused); break;
-    case 3: L = plan_agg_t<3, U>(n, inv_log_r, split, fused); break;
-    case 4: L = plan_agg_t<4, U>(n, inv_log_r, split, fused); break;
-    case 5: L = plan_agg_t<5, U>(n, inv_log_r, split, fused); break;
+    case 0: L = plan_agg_t<0, U>(n, inv_log_r, split, form); break;
+    case 1: L = plan_agg_t<1, U>(n, inv_log_r, split, form); break;
+    case 2: L = plan_agg_t<2, U>(n, inv_log_r, split, form); break;
+    case 3: L = plan_agg_t<3, U>(n, inv_log_r, split, form); break;
+    case 4: L = plan_agg_t<4, U>(n, inv_log_r, split, form); break;
+    case 5: L = plan_agg_t<5, U>(n, inv_log_r, split, form); break;
     default: return TilesShape{-1, 2};
     }
     return L.form == AggForm::Tiles ? TilesShape{L.items, L.diags} : TilesShape{-1, 2};
@@ -449,12 +450,11 @@ TilesShape aggregation_tiles_shape(const pbe_aggregation_params& p, int tile_u, 
                                    int form)
 {
     const int S = band_split_or_default(band_split);
-    const bool fused = form == 2;
     switch (tile_u) {
-    case 1: return tiles_shape_u<1>(p.n, p.kernel_type, p.inv_log_r, S, fused);
-    case 2: return tiles_shape_u<2>(p.n, p.kernel_type, p.inv_log_r, S, fused);
-    case 4: return tiles_shape_u<4>(p.n, p.kernel_type, p.inv_log_r, S, fused);
-    case 8: return tiles_shape_u<8>(p.n, p.kernel_type, p.inv_log_r, S, fused);
+    case 1: return tiles_shape_u<1>(p.n, p.kernel_type, p.inv_log_r, S, form);
+    case 2: return tiles_shape_u<2>(p.n, p.kernel_type, p.inv_log_r, S, form);
+    case 4: return tiles_shape_u<4>(p.n, p.kernel_type, p.inv_log_r, S, form);
+    case 8: return tiles_shape_u<8>(p.n, p.kernel_type, p.inv_log_r, S, form);
     default: return TilesShape{-1, 2};
     }
 }

@@ -11,11 +11,11 @@ pytestmark = pytest.mark.gpu
 DT = 0.5
 
 
-def _fit_problem(steps):
+def _fit_problem(steps, every=50):
     from pbe_hip import problems as P
     from pbe_hip.fit import BatchedFit
     inst, N0, x, t_q, bw_q = P.config_batched_fit(256, seed=256)
-    return BatchedFit(inst, N0, x, t_q, bw_q, DT, steps, every=50)
+    return BatchedFit(inst, N0, x, t_q, bw_q, DT, steps, every=every)
 
 
 def _theta_true(seed):
@@ -27,7 +27,8 @@ def test_batched_fit_gradient_vs_complex_step(pbe, oracle_mod):
     """The loss restricted (by its mask) to three of the 256 instances: its
     batched-adjoint gradient is the sum of those instances' complex-step
     gradients (d/d ln s = coef * d/d coef for the three scaled
-    coefficients, d/d alpha directly), to 1e-8 relative."""
+    coefficients, d/d alpha directly), to 1e-8 relative, with checkpoints
+    every 50 steps and with every stage state kept."""
     from pbe_hip import problems as P
     steps = 100
     F = _fit_problem(steps)
@@ -58,6 +59,12 @@ def test_batched_fit_gradient_vs_complex_step(pbe, oracle_mod):
     assert abs(L - Ls) <= 1e-10 * abs(Ls), (L, Ls)
     for p in range(4):
         assert abs(g[p] - expect[p]) <= 1e-8 * abs(expect[p]), (p, g, expect)
+    F0 = _fit_problem(steps, every=0)
+    F0.logT, F0.mask = F.logT, F.mask
+    L0, g0 = F0.loss_and_grad(F0.theta0, mask=mask)
+    assert abs(L0 - Ls) <= 1e-10 * abs(Ls), (L0, Ls)
+    for p in range(4):
+        assert abs(g0[p] - expect[p]) <= 1e-8 * abs(expect[p]), ("store all", p, g0, expect)
 
 
 def test_batched_joint_fit_converges(pbe):

@@ -117,6 +117,43 @@ def test_fused_rhs_every_kernel_and_selection(pbe, oracle_mod, kernel, selection
             assert err <= 1e-11, (ci, method, err)
 
 
+@pytest.mark.parametrize("n", [3072, 4096, 6144])
+def test_fused_rhs_mid_sizes(pbe, oracle_mod, n):
+    """Mid-size flocculation problems (Brownian+shear + threshold breakage,
+    r = 1.02) whose aggregation the tuned table runs as one tile launch of
+    128- or 256-column tiles also evaluate R_agg + R_brk in one launch under
+    the integrator: one Euler step per entry against N + dt (R_agg + R_brk)
+    of the oracle, and 2 RK4 steps against the oracle's RK4, with dt a
+    twentieth of max N / max |R| (the throughput workload's rates would blow
+    up a step of seconds)."""
+    from pbe_hip import hip, problems as P
+    x, N0, agg, brk, t_q, bw_q = P.config_flocculation(n)
+    kw = dict(agg=agg, brk=brk, t_q=t_q, bw_q=bw_q)
+    ra, Aa, _ = oracle_mod.aggregation_rhs(N0, x, threads=16, **agg)
+    rb, Ab, _ = oracle_mod.breakage_rhs(N0, x, t_q, bw_q, **brk)
+    dt = 0.05 * np.abs(N0).max() / np.abs(ra + rb).max()
+    print(n, pbe.aggregation_default_options(pbe.agg_params(n, agg)), dt)
+
+    it = pbe.Integrator(x, method=pbe.METHOD_EULER, **kw)
+    dN = hip.DeviceArray.from_host(N0)
+    hip.check(it.step(dN, dt, 1))
+    hip.synchronize()
+    expect = N0 + dt * (ra + rb)
+    tol = 1e-12 * dt * (Aa + Ab) + 1e-15 * np.abs(N0)
+    assert np.all(np.abs(dN.download() - expect) <= tol + 1e-300)
+
+    it = pbe.Integrator(x, method=pbe.METHOD_RK4, **kw)
+    dN = hip.DeviceArray.from_host(N0)
+    hip.check(it.step(dN, dt, 2))
+    hip.synchronize()
+    got = dN.download()
+    ref = oracle_mod.rk4(N0, x, dt, 2, threads=16, **kw)
+    assert np.all(np.isfinite(ref))
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print("rk4 2 steps:", err)
+    assert err <= 1e-11, err
+
+
 def test_euler_vs_oracle_one_step(pbe, oracle_mod):
     from pbe_hip import hip, problems as P
     x, N0, agg, brk, t_q, bw_q = P.config_flocculation(300)

@@ -279,6 +279,34 @@ def test_every_tile_shape(pbe, oracle_mod, u, n, kernel):
     assert_m1(got, x, A, D, what=f"U={u} n={n}")
 
 
+@pytest.mark.parametrize("n,r,kernel", [(4096, 1.02, 5), (6000, 1.01, 0), (3500, 1.03, 3),
+                                         (5000, 1.005, 5)])
+def test_tile_launch_form(pbe, oracle_mod, n, r, kernel):
+    """Launch form 3 (one launch of every tile, the tile diagonals within the
+    near-pair reach as row slices) at mid sizes, where the tuned table picks
+    it, at every tile width and two band splits."""
+    from pbe_hip import hip, problems as P
+    x = P.geometric_grid(n, 1e-18, r)
+    N = P.lognormal_ic(x, x[n // 4], 0.8, 1e12, noise=0.05, seed=n + kernel)
+    log_x0, inv_log_r = P.grid_lookup(x)
+    agg = dict(kernel=kernel, beta0=1e-17 if kernel != 2 else 1e-17 / x[n // 4],
+               beta_br=6.73e-18, beta_sh=4.0 / 3.0 * 100.0, log_x0=log_x0, inv_log_r=inv_log_r)
+    ref, A, D = _oracle(oracle_mod, N, x, agg=agg, threads=ORACLE_THREADS)
+    dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
+    drhs = hip.DeviceArray(n)
+    p = pbe.agg_params(n, agg)
+    for u in (1, 2, 4, 8):
+        for split in (2, 8):
+            drhs.zero()
+            hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=u, band_split=split,
+                                                    form=3))
+            hip.synchronize()
+            got = drhs.download()
+            what = f"form 3 U={u} split {split} n={n} r={r} kernel {kernel}"
+            assert_parity(got, ref, A, what=what)
+            assert_m1(got, x, A, D, what=what)
+
+
 @pytest.mark.parametrize("r", [1.005, 1.002])
 @pytest.mark.parametrize("u,n", [(1, 384), (1, 1500), (2, 1500), (4, 900)])
 @pytest.mark.parametrize("kernel", [0, 5])
