@@ -439,8 +439,11 @@ def main():
     # the launches the library's tuned shape makes for this call
     tu, tw, tf = pbe_hip.aggregation_default_options(ap)
     kname = pbe_hip.AggregationKernel(int(agg["kernel"])).name
-    if tu == 4 and tf == 2 and -(-n // 256) <= 64:
-        agg_label = f"aggregation call: agg_fused_kernel<{kname}> (band split {tw})"
+    if tu == 4 and tf in (2, 4) and -(-n // 256) <= 64:
+        agg_label = (f"aggregation call: agg_fused_kernel<{kname}> (band split {tw}, "
+                     f"{3 if tf == 4 else 2} workgroups per CU)")
+    elif tf == 3:
+        agg_label = f"aggregation call: agg_tiles_kernel<{kname}, {tu}> (band split {tw})"
     elif tu == 4:
         agg_label = (f"aggregation call: agg_sweep_kernel<{kname}> (band split {tw}) + "
                      f"agg_strip_kernel<{kname}, 4>")

@@ -97,15 +97,16 @@ hipError_t pbe_launch_aggregation_rhs(const double* N, const double* x,
  * launch of every tile when the strips would hold too little work), 2: one
  * launch running both side by side (taken for tile_u = 4 up to 64 column
  * blocks, otherwise the same as 1), 3: one launch of every tile (the
- * diagonal tiles within the near-pair reach as row slices).  0 picks the
- * library's autotuned choice
+ * diagonal tiles within the near-pair reach as row slices), 4: as 2 with
+ * three workgroups resident per CU instead of two.  0 picks the library's
+ * autotuned choice
  * for (n, kernel_type, grid reach), where the reach of a geometric grid
  * (inv_log_r > 0) is log_r(1/(r-1)) bins: how far below the diagonal a pair
  * can still land beyond its bracket (k, k+1). */
 typedef struct pbe_agg_options {
     int tile_u;       /* 0, 1, 2, 4 or 8 */
     int band_split;   /* 0, 1, 2, 4, 8 or 16 */
-    int form;         /* 0, 1, 2 or 3 */
+    int form;         /* 0, 1, 2, 3 or 4 */
 } pbe_agg_options;
 
 /* pbe_launch_aggregation_rhs with explicit launch options (same results). */

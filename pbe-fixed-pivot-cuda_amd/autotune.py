@@ -10,8 +10,8 @@ bucket, every block width 64*U (U = 1, 2, 4, 8 columns per lane) and band
 split S = 1, 2, 4, 8, 16 (row slices per diagonal-band tile; offset segments
 per block for the 256-wide band sweeps) is timed as two launches (band,
 then strips; F = 1) and as one launch of every tile (F = 3), at U = 4 also
-as the one fused band + strip launch (F = 2), and the fastest (U, S, F) is
-written.
+as the one fused band + strip launch at two or three workgroups per CU
+(F = 2, 4), and the fastest (U, S, F) is written.
 
 Batched launches (csrc/batch_tile_table.inc, rows {n_max, kernel_type,
 reach_max, U}): for every kernel, reach class and instance-size bucket, a
@@ -168,7 +168,7 @@ def tune_single(pbe, hip, timer, sizes, kernels, log):
                                   f"n {n} kernel {k} tile {u} split {w} form {f}")
                     return timer(launch)
                 t = {(u, w, f): run(u, w, f) for u in TILES for w in SPLITS
-                     for f in ((1, 2, 3) if u == 4 else (1, 3))}
+                     for f in ((1, 2, 3, 4) if u == 4 else (1, 3))}
                 bu, bw, bf = min(t, key=t.get)
                 best.setdefault((k, reach_max), []).append((n, t))
                 line.append(f"k{k}:U{bu}S{bw}F{bf} {t[(bu, bw, bf)] * 1e3:.1f}us")
@@ -316,7 +316,8 @@ def main():
                      "MI355X (gfx950); do not edit."],
                     merged_rows(best))
         summary += ["Single launches: chosen U (columns per lane), S (band split), F "
-                    "(launch form: 1 two launches, 2 fused band + strips, 3 one tile launch) "
+                    "(launch form: 1 two launches, 2 fused band + strips, 3 one tile launch, "
+                    "4 fused at three workgroups per CU) "
                     "and the time of one aggregation call in us.", ""]
         summary += summary_lines(best, batch=False)
     if args.part in ("batch", "both"):

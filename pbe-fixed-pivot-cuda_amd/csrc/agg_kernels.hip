@@ -183,7 +183,12 @@ void agg_strip_kernel(AggArgs a, StripPlan plan)
 // resident at once.  Whether this beats two launches depends on n and on the
 // grid's near-pair reach (measured: n = 16384, r = 1.02, Brownian+shear
 // 56.0 us against 62.4; at r = 1.01 114 against 91), so the tuned table
-// picks it.
+// picks it.  MB workgroups are resident per CU: 2 (form 2; 197 VGPRs, one
+// strip and one sweep wave per SIMD) or 3 (form 4; the compiler holds the
+// kernel to 168 VGPRs, spilling a few, so that three waves share a SIMD:
+// n = 16384, r = 1.02, Brownian+shear 52.6 us against 56.2, but slower at
+// n <= 8192, where the spills weigh more than the third wave: n = 4096 18.1
+// against 13.6 us).
 constexpr int kFusedWaves = 4;
 constexpr int kFusedMaxBlocks = 64;   // 256-column blocks (n <= 16384)
 
@@ -193,8 +198,8 @@ union FusedSmem {
     StripSmem<KT, kBandT / kWave, kFusedWaves> strip;
 };
 
-template <int KT>
-__global__ __launch_bounds__(kWave * kFusedWaves, 2)
+template <int KT, int MB>
+__global__ __launch_bounds__(kWave * kFusedWaves, MB)
 void agg_fused_kernel(AggArgs a, int nseg, BandSegs segs, StripPlan plan, int nsweep)
 {
     constexpr int U = kBandT / kWave;
@@ -221,7 +226,7 @@ void agg_fused_kernel(AggArgs a, int nseg, BandSegs segs, StripPlan plan, int ns
 }
 
 // Workgroups of the fused kernel resident on the current device at once.
-template <int KT>
+template <int KT, int MB>
 static int fused_resident_groups()
 {
     static std::atomic<int> cache[64];
@@ -234,7 +239,7 @@ static int fused_resident_groups()
         cus <= 0)
         cus = 256;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(&agg_fused_kernel<KT>), kWave * kFusedWaves,
+            &per_cu, reinterpret_cast<const void*>(&agg_fused_kernel<KT, MB>), kWave * kFusedWaves,
             0) != hipSuccess || per_cu <= 0)
         per_cu = 1;
     v = cus * per_cu;
@@ -299,10 +304,12 @@ static StripPlan make_strip_plan(int n, int U, bool band, double inv_log_r)
 // Which launch(es) an aggregation call makes at block width U (the tuned
 // band split and form given): the fused band + strip launch, one launch of
 // every tile, or a band launch followed by the strip launch.  Form 3 asks
-// for the tile launch whatever the strips' share of the work.
+// for the tile launch whatever the strips' share of the work; form 4 is the
+// fused launch at three workgroups per CU.
 enum class AggForm { Fused, Tiles, BandStrips };
 struct AggLaunch {
     AggForm form;
+    int mb;              // Fused: workgroups per CU the kernel is built for (2 or 3)
     int nseg;            // Fused: offset segments per block
     long long groups;    // Fused / BandStrips: resident workgroups of the strip part
     long long items;     // Tiles: work items; Fused: sweep items; BandStrips: band items
@@ -329,10 +336,12 @@ static AggLaunch plan_agg_t(int n, double inv_log_r, int split, int form)
     AggLaunch L;
     L.sep = make_strip_plan(n, U, false, inv_log_r);
     L.nseg = split;
-    if (form == 2 && T == kBandT && nb <= kFusedMaxBlocks && L.sep.total > 0) {
+    L.mb = form == 4 ? 3 : 2;
+    if ((form == 2 || form == 4) && T == kBandT && nb <= kFusedMaxBlocks && L.sep.total > 0) {
         // mid sizes: sweep items on at most half of the resident slots of
         // the fused kernel, strip shares on the rest (empty shares return)
-        const long long Gf = fused_resident_groups<KT>();
+        const long long Gf = L.mb == 3 ? fused_resident_groups<KT, 3>()
+                                       : fused_resident_groups<KT, 2>();
         int Sf = split;
         while (Sf > 1 && (long long)nb * Sf * 2 > Gf) Sf /= 2;
         const long long nsweep = (long long)nb * Sf;
@@ -369,9 +378,16 @@ static hipError_t launch_agg_t(AggArgs args, int split, int form, hipStream_t st
     if (L.items > 0x7fffffffLL) return hipErrorInvalidValue;
     switch (L.form) {
     case AggForm::Fused:
-        hipLaunchKernelGGL((agg_fused_kernel<KT>), dim3((unsigned)L.groups),
-                           dim3(kWave * kFusedWaves), 0, stream, args, L.nseg,
-                           band_segments_cached(L.nseg, args.inv_log_r), L.sep, (int)L.items);
+        if (L.mb == 3)
+            hipLaunchKernelGGL((agg_fused_kernel<KT, 3>), dim3((unsigned)L.groups),
+                               dim3(kWave * kFusedWaves), 0, stream, args, L.nseg,
+                               band_segments_cached(L.nseg, args.inv_log_r), L.sep,
+                               (int)L.items);
+        else
+            hipLaunchKernelGGL((agg_fused_kernel<KT, 2>), dim3((unsigned)L.groups),
+                               dim3(kWave * kFusedWaves), 0, stream, args, L.nseg,
+                               band_segments_cached(L.nseg, args.inv_log_r), L.sep,
+                               (int)L.items);
         return hipSuccess;
     case AggForm::Tiles:
         hipLaunchKernelGGL((agg_tiles_kernel<KT, U>), dim3((unsigned)L.items), dim3(256), 0,

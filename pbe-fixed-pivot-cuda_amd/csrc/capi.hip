@@ -117,7 +117,7 @@ extern "C" hipError_t pbe_launch_aggregation_rhs_ex(const double* N, const doubl
     if (f == 0) f = t.f;
     if (u != 1 && u != 2 && u != 4 && u != 8) return hipErrorInvalidValue;
     if (w != 1 && w != 2 && w != 4 && w != 8 && w != 16) return hipErrorInvalidValue;
-    if (f < 1 || f > 3) return hipErrorInvalidValue;
+    if (f < 1 || f > 4) return hipErrorInvalidValue;
     hipError_t e = pbe::launch_aggregation(N, x, rhs, *p, u, w, f, stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();

@@ -10,7 +10,7 @@ namespace pbe {
 // Enqueue the aggregation tiles (U = columns per lane: 2, 4 or 8).
 // form 2: band and strips in one launch where the shape allows it (tile_u
 // = 4, at most 64 column blocks); 1: two launches; 3: one launch of every
-// tile.
+// tile; 4: as 2, three workgroups per CU.
 hipError_t launch_aggregation(const double* N, const double* x, double* rhs,
                               const pbe_aggregation_params& p, int tile_u, int band_split,
                               int form, hipStream_t stream);

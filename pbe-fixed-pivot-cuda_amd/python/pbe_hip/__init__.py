@@ -54,8 +54,8 @@ class AggregationParams(ctypes.Structure):
 class AggOptions(ctypes.Structure):
     """pbe_agg_options: block width (columns per lane), row slices per
     diagonal-band tile and launch form (1: band and strips apart, 2: one
-    launch of both, 3: one launch of every tile); 0 selects the autotuned
-    value."""
+    launch of both, 3: one launch of every tile, 4: as 2 at three
+    workgroups per CU); 0 selects the autotuned value."""
     _fields_ = [("tile_u", ctypes.c_int), ("band_split", ctypes.c_int),
                 ("form", ctypes.c_int)]
 

@@ -176,10 +176,10 @@ def test_deterministic_argument_errors(pbe):
 
 def test_aggregation_option_errors(pbe):
     """tile_u outside {0,1,2,4,8}, band_split outside {0,1,2,4,8,16} or form
-    outside {0,1,2,3} are refused."""
+    outside {0,1,2,3,4} are refused."""
     lib = pbe.library()
     A = pbe.AggregationParams(n=100, kernel_type=5)
-    for u, w, f in ((3, 0, 0), (16, 0, 0), (4, 3, 0), (4, 32, 0), (4, -1, 0), (4, 4, 4),
+    for u, w, f in ((3, 0, 0), (16, 0, 0), (4, 3, 0), (4, 32, 0), (4, -1, 0), (4, 4, 5),
                     (4, 4, -1)):
         opt = pbe.AggOptions(u, w, f)
         assert lib.pbe_launch_aggregation_rhs_ex(FAKE, FAKE, FAKE, ctypes.byref(A),
@@ -205,7 +205,7 @@ def test_tile_tables_cover_every_case(name, width):
     for r in rows:
         assert r[1] in range(-1, 6) and r[3] in (1, 2, 4, 8)
         if width == 6:
-            assert r[4] in (1, 2, 4, 8, 16) and r[5] in (1, 2, 3)
+            assert r[4] in (1, 2, 4, 8, 16) and r[5] in (1, 2, 3, 4)
         groups.setdefault((r[1], r[2]), []).append(r[0])
     for key, nmax in groups.items():
         assert nmax == sorted(nmax) and len(set(nmax)) == len(nmax), key

@@ -389,12 +389,12 @@ def test_strips_fine_grid(pbe, oracle_mod, r, kernel):
 
 @pytest.mark.parametrize("kernel", range(6))
 def test_fused_sweep_and_strips(pbe, oracle_mod, kernel):
-    """Launch form 2 (at most 64 blocks of 256 columns): the band sweep items
-    and the four-wave strip shares in one launch, for every band split (the
-    launch halves it while the sweep would take over half the slots), on
-    geometric grids (start-bracket offsets; r = 1.01 puts near chunks in the
-    strips) and a jittered one (searches); n = 16385 (65 blocks) takes two
-    launches."""
+    """Launch forms 2 and 4 (at most 64 blocks of 256 columns; two or three
+    workgroups per CU): the band sweep items and the four-wave strip shares
+    in one launch, for every band split (the launch halves it while the
+    sweep would take over half the slots), on geometric grids (start-bracket
+    offsets; r = 1.01 puts near chunks in the strips) and a jittered one
+    (searches); n = 16385 (65 blocks) takes two launches."""
     from pbe_hip import hip, problems as P
     for n, jitter, r in ((3400, 0.0, 1.02), (5000, 0.0, 1.02), (8192, 0.0, 1.02),
                          (6000, 0.2, 1.02), (12000, 0.0, 1.01), (16384, 0.0, 1.02),
@@ -409,13 +409,13 @@ def test_fused_sweep_and_strips(pbe, oracle_mod, kernel):
         dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
         drhs = hip.DeviceArray(n)
         p = pbe.agg_params(n, agg)
-        for split in (1, 4, 16):
+        for split, form in ((1, 2), (4, 2), (16, 2), (2, 4), (16, 4)):
             drhs.zero()
             hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=4, band_split=split,
-                                                    form=2))
+                                                    form=form))
             hip.synchronize()
             got = drhs.download()
-            what = f"fused kernel {kernel} n={n} r={r} jitter {jitter} split {split}"
+            what = f"fused form {form} kernel {kernel} n={n} r={r} jitter {jitter} split {split}"
             assert_parity(got, ref, A, what=what)
             assert_m1(got, x, A, D, what=what)
 
