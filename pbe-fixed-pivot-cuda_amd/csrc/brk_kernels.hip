@@ -7,7 +7,7 @@
 // n-1 for v >= x_{n-1}, otherwise split between the left bracket lo and lo+1
 // with eta = (v - x_lo)/(x_{lo+1} - x_lo)).
 //
-// Design: one launch does death and birth.  A 256-thread workgroup owns P
+// Design: one launch does death and birth.  A 1024-thread workgroup owns P
 // consecutive parents; S_j N_j is evaluated once per parent (one pow per
 // parent for the power law, not one per quadrature point), the fragments of
 // those parents land in the window of bins just below them, which is staged
@@ -20,12 +20,17 @@
 
 namespace pbe {
 
+// 1024 threads per block of 256 parents: a table of m quadrature points
+// gives each thread m / 4 birth items (dependent chains of LDS reads and a
+// divide) instead of m, and each SIMD four waves to switch between.
+constexpr int kBrkThreads = 1024;
+
 template <int SEL>
-__global__ __launch_bounds__(256)
+__global__ __launch_bounds__(kBrkThreads)
 void brk_kernel(BrkArgs a)
 {
     __shared__ BrkSmem sm;
-    brk_block_body<SEL>(a, blockIdx.x * kBrkParents, sm);
+    brk_block_body<SEL, kBrkThreads>(a, blockIdx.x * kBrkParents, sm);
 }
 
 hipError_t launch_breakage(const double* N, const double* x, const double* t_q,
@@ -35,10 +40,10 @@ hipError_t launch_breakage(const double* N, const double* x, const double* t_q,
     BrkArgs args{N, x, t_q, bw_q, rhs, p.n, p.n_quad, p.S0, p.v_ref, p.alpha, p.v_min};
     const unsigned grid = (unsigned)((p.n + kBrkParents - 1) / kBrkParents);
     switch (p.selection) {
-    case 0: hipLaunchKernelGGL(brk_kernel<0>, dim3(grid), dim3(256), 0, stream, args); break;
-    case 1: hipLaunchKernelGGL(brk_kernel<1>, dim3(grid), dim3(256), 0, stream, args); break;
-    case 2: hipLaunchKernelGGL(brk_kernel<2>, dim3(grid), dim3(256), 0, stream, args); break;
-    case 3: hipLaunchKernelGGL(brk_kernel<3>, dim3(grid), dim3(256), 0, stream, args); break;
+    case 0: hipLaunchKernelGGL(brk_kernel<0>, dim3(grid), dim3(kBrkThreads), 0, stream, args); break;
+    case 1: hipLaunchKernelGGL(brk_kernel<1>, dim3(grid), dim3(kBrkThreads), 0, stream, args); break;
+    case 2: hipLaunchKernelGGL(brk_kernel<2>, dim3(grid), dim3(kBrkThreads), 0, stream, args); break;
+    case 3: hipLaunchKernelGGL(brk_kernel<3>, dim3(grid), dim3(kBrkThreads), 0, stream, args); break;
     default: return hipErrorInvalidValue;
     }
     return hipSuccess;
