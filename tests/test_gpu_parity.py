@@ -119,6 +119,39 @@ def test_breakage_jittered(pbe, oracle_mod, selection, daughter, nq):
     assert_m1(got, x, A, D, what=f"sel {selection} {daughter}{nq}")
 
 
+@pytest.mark.parametrize("nq", [3, 7, 24, 100])
+@pytest.mark.parametrize("grid", ["geometric", "jittered"])
+def test_breakage_quadrature_sizes(pbe, oracle_mod, nq, grid):
+    """Uniform daughter tables whose point count does not divide the
+    breakage launch's thread count (a thread's items then change quadrature
+    point, and the bracket cursor restarts) or exceeds the 64 points held in
+    LDS (read from global memory), linear selection so that every parent
+    breaks, on a geometric grid (the cursor's guesses hold) and a jittered
+    one (they miss and the search runs); the plain call and the batched form
+    (256-thread breakage blocks) against the oracle."""
+    from pbe_hip import hip, problems as P
+    n = 3000
+    x = (P.geometric_grid(n, 1e-18, 1.01) if grid == "geometric"
+         else P.jittered_grid(n, 1e-18, 1.01, 0.2, seed=nq))
+    N = P.lognormal_ic(x, x[n // 2], 3.0, 1e12, noise=0.05, seed=nq + 1)
+    t_q, bw_q = P.daughter_uniform(nq)
+    brk = dict(selection=1, S0=1e-3, v_ref=x[n // 2], alpha=1.0, v_min=0.0)
+    ref, A, D = _oracle(oracle_mod, N, x, brk=brk, t_q=t_q, bw_q=bw_q)
+    got = _gpu(pbe, N, x, brk=brk, t_q=t_q, bw_q=bw_q)
+    assert_parity(got, ref, A, what=f"{grid} n_quad {nq}")
+    assert_m1(got, x, A, D, what=f"{grid} n_quad {nq}")
+    inst = [dict(n=n, offset=0, S0=brk["S0"], v_ref=brk["v_ref"], alpha=brk["alpha"],
+                 v_min=brk["v_min"])]
+    plan = pbe.BatchPlan(inst, kernel_type=-1, selection=1, n_quad=nq)
+    dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
+    dt, db = hip.DeviceArray.from_host(t_q), hip.DeviceArray.from_host(bw_q)
+    drhs = hip.DeviceArray(n)
+    drhs.zero()
+    hip.check(plan.launch(dN, dx, dt, db, drhs), "pbe_launch_batched_rhs")
+    hip.synchronize()
+    assert_parity(drhs.download(), ref, A, what=f"batched {grid} n_quad {nq}")
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 127, 129, 255, 257, 511, 513, 1000])
 def test_ragged_sizes(pbe, oracle_mod, n):
     """Sizes around the tile edges, including n = 1 and 2 where every pair clips."""
