@@ -75,7 +75,6 @@ struct AggSmem {
     double rowwin[AggShape<U>::T];      // row deaths of an off-diagonal tile
     double colc[AggShape<U>::T];        // cube roots of the tile's column pivots
     double colic[AggShape<U>::T];       // and their reciprocals
-    double red[4][R];
 };
 
 // Diagonal-major tile index t -> (position p along diagonal d) for nb blocks.
@@ -242,7 +241,6 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
     constexpr int WIN = S::WIN;
     auto& rowbuf = sm.rowbuf;
     double* colwin = sm.colwin;
-    auto& red = sm.red;
 
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
@@ -371,6 +369,8 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
     // columns) bin i loses sum_{k in block} beta_ik N_i N_k = sum_r A_r(i) TK_r,
     // which also covers the diagonal pair (2 x rate/2).  Summed over all tiles
     // this is the full death N_i sum_k beta_ik N_k of the reference.
+    // Each wave takes the column deaths of its own rows (sum_r B_r(k) tj_r,
+    // tj_r over the wave's rows), so no wave waits for another's sums.
     {
         double tk[R];
 #pragma unroll
@@ -379,12 +379,7 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
 #pragma unroll
             for (int u = 0; u < U; ++u) sacc += Bc[u][r];
             tk[r] = wave_sum(sacc);
-            if (lane == 0) red[wv][r] = tj[r];
         }
-        __syncthreads();
-        double TJ[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) TJ[r] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
         for (int rr = lane; rr < rpw; rr += kWave) {
             const int j = rw0 + rr;
             if (j < n) {
@@ -395,14 +390,14 @@ __device__ __forceinline__ void agg_tile_body(const AggArgs& a, const int j0, co
                 else sm.rowwin[j - j0] -= dth;   // rows of this wave only: no race
             }
         }
-        if (!diag && wv == 0) {
+        if (!diag && rw0 < rw1) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int k = kc[u];
                 if (k < n) {
                     double dth = 0.0;
 #pragma unroll
-                    for (int r = 0; r < R; ++r) dth = fma(Bc[u][r], TJ[r], dth);
+                    for (int r = 0; r < R; ++r) dth = fma(Bc[u][r], tj[r], dth);
                     lds_add(&colwin[k - k0], -dth);
                 }
             }
