@@ -11,7 +11,10 @@ Prints one JSON line.  ``roofline`` refers to the dominant work, the
 aggregation call (at n = 16384 one fused launch of the diagonal-band sweep
 items and the column strips): achieved = 21 FP64 FLOP per pair x pairs /
 average duration of the call, measured with HIP events on the launch stream
-around it; beside it the FP64 FLOPs the call executes (rocprofv3
+around it in a second pass over the same steps (the timed region carries
+events only at its two ends: events between launches add several
+microseconds to each step); beside it the rate over K launches back to back,
+and the FP64 FLOPs the call executes (rocprofv3
 SQ_INSTS_VALU_FLOPS_FP64 of the same launch, committed under profiles/) over
 the same time.
 ``cpu_baseline`` times the OpenMP FP64 restatement of the reference
@@ -393,28 +396,50 @@ def main():
         step()
     stream.synchronize()
 
-    # timed region: K steps, events on the launch stream around each
-    # aggregation launch give that kernel's average duration
+    # timed region: K steps back to back on the launch stream, events only
+    # at its two ends (an event recorded between two launches adds several
+    # microseconds of its own to every step)
     K = args.steps
-    ev = [(hip.Event(), hip.Event()) for _ in range(K)]
     e0, e1 = hip.Event(), hip.Event()
     hip.synchronize()
     t0 = time.perf_counter()
     e0.record(stream)
+    for _ in range(K):
+        step()
+    e1.record(stream)
+    stream.synchronize()
+    hip.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = e0.elapsed_ms(e1) / K
+
+    # the aggregation launch's own duration: the same K steps again, with
+    # events on the launch stream around each aggregation call
+    ev = [(hip.Event(), hip.Event()) for _ in range(K)]
+    g0, g1 = hip.Event(), hip.Event()
+    g0.record(stream)
     for i in range(K):
         hip.check(rt.hipMemsetAsync(drhs.ptr, 0, drhs.nbytes, stream.handle), "memset")
         ev[i][0].record(stream)
         hip.check(pbe_hip.launch_aggregation_rhs(dN, dx, drhs, ap, stream), "agg")
         ev[i][1].record(stream)
         hip.check(pbe_hip.launch_breakage_rhs(dN, dx, dt, db, drhs, bp, stream), "brk")
-    e1.record(stream)
+    g1.record(stream)
     stream.synchronize()
-    hip.synchronize()
-    wall = time.perf_counter() - t0
-    ms_step = e0.elapsed_ms(e1) / K
+    ms_step_events = g0.elapsed_ms(g1) / K
     agg_each = sorted(a.elapsed_ms(b) for a, b in ev)
     agg_ms = sum(agg_each) / K
     agg_median = agg_each[K // 2] if K % 2 else 0.5 * (agg_each[K // 2 - 1] + agg_each[K // 2])
+
+    # K aggregation launches back to back between two events (rhs
+    # accumulates; timing only): the launch's duration without the events'
+    # own cost
+    b0, b1 = hip.Event(), hip.Event()
+    b0.record(stream)
+    for _ in range(K):
+        hip.check(pbe_hip.launch_aggregation_rhs(dN, dx, drhs, ap, stream), "agg")
+    b1.record(stream)
+    stream.synchronize()
+    agg_ms_b2b = b0.elapsed_ms(b1) / K
 
     # same K steps without the per-step memset (rhs accumulates; timing only)
     f0, f1 = hip.Event(), hip.Event()
@@ -501,7 +526,10 @@ def main():
             if executed else None,
             "avg_launch_ms": agg_ms,
             "median_launch_ms": agg_median,
+            "avg_launch_ms_back_to_back": agg_ms_b2b,
+            "frac_back_to_back": flops / (agg_ms_b2b * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
         },
+        "ms_per_step_with_launch_events": ms_step_events,
         "ms_per_step_without_memset": ms_step_nomemset,
         "spot_check": spot,
         "wall_s_timed_region": wall,
