@@ -9,8 +9,9 @@ n(n+1)/2 pairs + n*n_quad breakage updates (dense: pairs with N = 0 count).
 
 Prints one JSON line.  ``roofline`` refers to the dominant work, the
 aggregation call (at n = 16384 one fused launch of the diagonal-band sweep
-items and the column strips): achieved = 21 FP64 FLOP per pair x pairs /
-average duration of the call, K calls back to back between two HIP events on
+items and the column strips): achieved = 21 FP64 FLOP per pair whose birth
+weight is not exactly zero (fl(x_j + x_k) != x_k; the rest only lose their
+rate from bin j, a separable O(n) death) / average duration of the call, K calls back to back between two HIP events on
 the launch stream (the figure rocprofv3's kernel trace of the launch agrees
 with); beside it the average of HIP events around each call in a second pass
 over the same steps (events between launches add their own cost, so the
@@ -460,7 +461,12 @@ def main():
     stream.synchronize()
     spot = spot_check(drhs.download(), N, x, agg, brk, t_q, bw_q)
 
-    flops = P.PAIR_FLOPS[int(agg["kernel"])] * pairs
+    # algorithmic FLOPs: 21 per pair whose birth weight is not exactly zero
+    # (fl(x_j + x_k) != x_k); the other pairs only move their rate out of
+    # bin j, a separable death (O(n) for all of them), and are left out
+    bpairs = P.birth_pairs(x)
+    flops = P.PAIR_FLOPS[int(agg["kernel"])] * bpairs
+    dense_flops = P.PAIR_FLOPS[int(agg["kernel"])] * pairs
     # the launch's average duration: K launches back to back between two
     # events (agrees with rocprofv3's kernel trace of the same launch; events
     # around each call add their own cost to every reading)
@@ -524,6 +530,9 @@ def main():
             "traffic": traffic,
             "kernel": agg_label,
             "flop_per_launch": flops,
+            "birth_pairs": bpairs,
+            "dense_flop_per_launch": dense_flops,
+            "dense_tflops": dense_flops / (agg_ms_b2b * 1e-3) / 1e12,
             "executed_fp64_flop_per_launch": executed,
             "executed_tflops": (executed / (agg_ms_b2b * 1e-3) / 1e12) if executed else None,
             "executed_frac": (executed / (agg_ms_b2b * 1e-3) / 1e12 / FP64_PEAK_TFLOPS)

@@ -38,6 +38,7 @@
 //
 //  * One flush per tile or visit of a block: the LDS window is added to rhs
 //    with FP64 global atomics (zero entries skipped).
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -272,10 +273,10 @@ static int resident_groups()
 
 // Strip plan of an n-bin problem in blocks of T = 64 U columns (the rows
 // below each block's diagonal band; `band`: the band's rows too).  Chunk
-// costs on the workgroups' cost line: a far chunk 4, a near chunk 40 (a near
-// row is a chain of dependent steps with a bracket move in most of them: on
-// the MI355X it takes about ten times a far row), a diagonal chunk 6, a
-// block's visit 20.  On a geometric grid the strip chunks within the
+// costs on the workgroups' cost line: a zero-birth chunk 2, a far chunk 20,
+// a near chunk 200 (a near row is a chain of dependent steps with a bracket
+// move in most of them: on the MI355X it takes about ten times a far row), a
+// diagonal chunk 30, a block's visit 100.  On a geometric grid the strip chunks within the
 // near-pair reach log_r(1 / (r - 1)) bins below a block's band are the near
 // ones; measured on the strip launch at r = 1.005: n = 16384 231 us with the
 // near chunks charged 10, 107 us at 40; n = 65536 (512-column blocks)
@@ -293,10 +294,41 @@ static StripPlan make_strip_plan(int n, int U, bool band, double inv_log_r)
         if (reach > T && reach < 1e7) plan.m_near = (int)std::ceil((reach - T) / kStripRows);
     }
     plan.mq = plan.m_near / U + 1;
-    plan.cf = 4;
-    plan.cn = 40;
-    plan.cd = 6;
-    plan.cv = 20;
+    // Zero-birth chunks: on a geometric grid the rows more than
+    // log_r 2^53.5 bins below a block's first column lie below half an ulp
+    // of its pivot (the kernel tests every row; this only prices them).
+    // Such a chunk stages its rows and adds their deaths but runs no pair
+    // loop: priced at a tenth of a far chunk, and only where such chunks are
+    // at least 60 % of the far ones.  Measured on the MI355X (r = 1.02,
+    // Brownian+shear, default launch shape): n = 16384 49.4 us unpriced,
+    // 36.3 at a tenth, 36.0 at a fifth, 39.8 at two fifths (the strip launch
+    // alone: 43.9 at a whole far chunk, 34.5 at a twentieth, 28.9 at a
+    // tenth); n = 32768 154 -> 72 us; n = 12288 (74 % zero-birth) 41.6 ->
+    // 30.8; n = 8192 (62 %) 22.2 -> 21.9-22.9; n = 6144 (51 %) 21.0 -> 22.1-
+    // 23.5 and n = 4096 (30 %) 15.0 -> 17.6-18.3, where the chunks are too
+    // few for the price to pay.
+    plan.zb = 0;
+    plan.m_zero = 0;
+    plan.mz = 1;
+    plan.cz = 2;
+    if (inv_log_r > 0.0 && std::isfinite(inv_log_r)) {
+        const double dz = 53.5 * std::log(2.0) * inv_log_r;
+        if (dz < 1e7) {
+            plan.zb = 1;
+            plan.m_zero = std::max((int)std::ceil(dz / kStripRows), U + plan.m_near);
+            plan.mz = plan.m_zero / U + 1;
+            const long long zf = strip_sz_sum(plan, plan.nb), ff = strip_sf_sum(plan, plan.nb);
+            if (!(ff > 0 && 10 * zf >= 6 * ff)) {
+                plan.zb = 0;
+                plan.m_zero = 0;
+                plan.mz = 1;
+            }
+        }
+    }
+    plan.cf = 20;
+    plan.cn = 200;
+    plan.cd = 30;
+    plan.cv = 100;
     plan.total = strip_total(plan);
     return plan;
 }

@@ -46,9 +46,15 @@ constexpr int kStripExtra = 256;     // near-birth reach of the LDS window beyon
 // lower tile (rows [(q-1)T, qT)), then the diagonal tile (rows [qT, (q+1)T)).
 // Chunks [sf, se) are near chunks (the band's lower tile, and on fine grids
 // the m_near strip chunks just below it), chunks [se, nc) diagonal chunks;
-// the rest are far chunks.  The shares of the launch split a cost line on
-// which a far chunk costs cf, a near chunk cn, a diagonal chunk cd and each
-// block's visit (column set-up and window flush) cv.
+// the rest are far chunks, and of those the first sz are zero-birth chunks
+// (with `zb` set): rows so far below the block that x_j + x_k rounds to x_k
+// for every column k, so the pair's birth weight is exactly 0 and only the
+// row's death remains to be added (a geometric grid of ratio r has them
+// from about 54 log_r 2 bins below the block's first column down; sz(q) =
+// max(0, q U - m_zero) estimates them for the cost line, the kernel tests
+// every row).  The shares of the launch split a cost line on which a
+// zero-birth chunk costs cz, a far chunk cf, a near chunk cn, a diagonal
+// chunk cd and each block's visit (column set-up and window flush) cv.
 struct StripPlan {
     long long total;   // chunks of all blocks
     int nb;            // column blocks
@@ -56,7 +62,10 @@ struct StripPlan {
     int band;          // 1: blocks hold their diagonal band rows too
     int m_near;        // strip chunks below the band charged as near
     int mq;            // m_near / U + 1 (formed on the host)
-    int cf, cn, cd, cv;
+    int zb;            // 1: the first sz(q) chunks of block q are charged cz
+    int m_zero;        // chunks below the block's first column not charged cz (>= U + m_near)
+    int mz;            // m_zero / U + 1 (formed on the host)
+    int cz, cf, cn, cd, cv;
 };
 
 __host__ __device__ __forceinline__ int strip_nc(const StripPlan& s, int q)
@@ -75,6 +84,13 @@ __host__ __device__ __forceinline__ int strip_sf(const StripPlan& s, int q)
 {
     const int st = q >= 1 ? (q - 1) * s.U : 0;
     return st > s.m_near ? st - s.m_near : 0;
+}
+
+// first far chunk of block q charged cf (the chunks before it are charged cz)
+__host__ __device__ __forceinline__ int strip_sz(const StripPlan& s, int q)
+{
+    const int st = q * s.U - s.m_zero;
+    return s.zb && st > 0 ? st : 0;
 }
 
 __host__ __device__ __forceinline__ long long tri(long long m) { return m > 0 ? m * (m + 1) / 2 : 0; }
@@ -99,11 +115,22 @@ __host__ __device__ __forceinline__ long long strip_sf_sum(const StripPlan& s, i
            (long long)s.m_near * (A - a0 + 1);
 }
 
+__host__ __device__ __forceinline__ long long strip_sz_sum(const StripPlan& s, int q)
+{
+    // sum over q' in [0, q - 1] of max(0, U q' - m_zero)
+    if (!s.zb) return 0;
+    const long long A = q - 1, a0 = s.mz;
+    if (A < a0) return 0;
+    return (long long)s.U * (A * (A + 1) / 2 - a0 * (a0 - 1) / 2) -
+           (long long)s.m_zero * (A - a0 + 1);
+}
+
 __host__ __device__ __forceinline__ long long strip_cost_start(const StripPlan& s, int q)
 {
     const long long nc = strip_prefix(s, q), se = strip_se_sum(s, q), sf = strip_sf_sum(s, q);
+    const long long sz = strip_sz_sum(s, q);
     const long long visits = s.band ? q : (q > 2 ? q - 2 : 0);
-    return s.cv * visits + s.cf * sf + s.cn * (se - sf) + s.cd * (nc - se);
+    return s.cv * visits + s.cz * sz + s.cf * (sf - sz) + s.cn * (se - sf) + s.cd * (nc - se);
 }
 
 __host__ __device__ __forceinline__ long long strip_total(const StripPlan& s)
@@ -128,18 +155,19 @@ __host__ __device__ __forceinline__ long long ceil_div(long long a, long long b)
 // Cost of block q's visit and chunks.
 __host__ __device__ __forceinline__ long long strip_block_cost(const StripPlan& s, int q)
 {
-    const int nc = strip_nc(s, q), se = strip_se(s, q), sf = strip_sf(s, q);
-    return (nc > 0 ? s.cv : 0) + (long long)s.cf * sf + (long long)s.cn * (se - sf) +
-           (long long)s.cd * (nc - se);
+    const int nc = strip_nc(s, q), se = strip_se(s, q), sf = strip_sf(s, q), sz = strip_sz(s, q);
+    return (nc > 0 ? s.cv : 0) + (long long)s.cz * sz + (long long)s.cf * (sf - sz) +
+           (long long)s.cn * (se - sf) + (long long)s.cd * (nc - se);
 }
 
 // Block whose cost range holds f (largest q < nb with cost_start(q) <= f).
-// Past the first few blocks a block costs A + B q with B = cf U, so the
+// Past the first few blocks a block costs A + B q with B = cf U (cz U once
+// zero-birth chunks are charged), so the
 // start of block q is close to A q + B q (q - 1) / 2: a square root gives
 // the block to within a step or two, then exact comparisons settle it.
 __host__ __device__ __forceinline__ int strip_block_at_cost(const StripPlan& s, long long f)
 {
-    const double B = (double)s.cf * s.U;
+    const double B = (double)(s.zb ? s.cz : s.cf) * s.U;
     const int qr = s.nb - 1;
     const double A = (double)strip_block_cost(s, qr) - B * qr;
     const double b = A - 0.5 * B;
@@ -173,17 +201,21 @@ __host__ __device__ __forceinline__ void strip_chunk_at_cost(const StripPlan& s,
         return;
     }
     q = strip_block_at_cost(s, f);
-    const int nc = strip_nc(s, q), se = strip_se(s, q), sf = strip_sf(s, q);
+    const int nc = strip_nc(s, q), se = strip_se(s, q), sf = strip_sf(s, q), sz = strip_sz(s, q);
     const long long off = f - strip_cost_start(s, q) - (nc > 0 ? s.cv : 0);
+    const long long e0 = (long long)s.cz * sz, e1 = e0 + (long long)s.cf * (sf - sz),
+                    e2 = e1 + (long long)s.cn * (se - sf);
     long long cc;
     if (off <= 0) {
         cc = 0;
-    } else if (off <= (long long)s.cf * sf) {
-        cc = ceil_div(off, s.cf);
-    } else if (off <= (long long)s.cf * sf + (long long)s.cn * (se - sf)) {
-        cc = sf + ceil_div(off - (long long)s.cf * sf, s.cn);
+    } else if (off <= e0) {
+        cc = ceil_div(off, s.cz);
+    } else if (off <= e1) {
+        cc = sz + ceil_div(off - e0, s.cf);
+    } else if (off <= e2) {
+        cc = sf + ceil_div(off - e1, s.cn);
     } else {
-        cc = se + ceil_div(off - (long long)s.cf * sf - (long long)s.cn * (se - sf), s.cd);
+        cc = se + ceil_div(off - e2, s.cd);
     }
     if (cc >= nc) {
         q += 1;
@@ -192,34 +224,41 @@ __host__ __device__ __forceinline__ void strip_chunk_at_cost(const StripPlan& s,
     c = (int)cc;
 }
 
-// Rows of a visit of block q split by cost: row j weighs cf below row 64 sf,
-// cn below row 64 se and cd above.  row_at(t) is the first row whose cost
-// offset from r0 is at least t; cost(j) the offset of row j.
+// Rows of a visit of block q split by cost: row j weighs cz below row 64 sz,
+// cf below row 64 sf, cn below row 64 se and cd above.  row_at(t) is the
+// first row whose cost offset from r0 is at least t; cost(j) the offset of
+// row j.
 struct VisitRows {
-    int r0, r1, b1, b2, wf, wn, wd, c1, c2;
+    int r0, r1, b0, b1, b2, wz, wf, wn, wd, c0, c1, c2;
     __host__ __device__ static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
     __host__ __device__ static int ceil_i(int a, int b) { return (int)ceil((double)a / (double)b); }
     __host__ __device__ __forceinline__ VisitRows(const StripPlan& s, int q, int r0_, int r1_)
     {
         r0 = r0_;
         r1 = r1_;
+        b0 = clampi(kStripRows * strip_sz(s, q), r0, r1);
         b1 = clampi(kStripRows * strip_sf(s, q), r0, r1);
         b2 = clampi(kStripRows * strip_se(s, q), r0, r1);
+        wz = s.cz;
         wf = s.cf;
         wn = s.cn;
         wd = s.cd;
-        c1 = wf * (b1 - r0);
+        c0 = wz * (b0 - r0);
+        c1 = c0 + wf * (b1 - b0);
         c2 = c1 + wn * (b2 - b1);
     }
     __host__ __device__ __forceinline__ int total() const { return c2 + wd * (r1 - b2); }
     __host__ __device__ __forceinline__ int cost(int j) const
     {
-        return j <= b1 ? wf * (j - r0) : (j <= b2 ? c1 + wn * (j - b1) : c2 + wd * (j - b2));
+        return j <= b0 ? wz * (j - r0)
+                       : (j <= b1 ? c0 + wf * (j - b0)
+                                  : (j <= b2 ? c1 + wn * (j - b1) : c2 + wd * (j - b2)));
     }
     __host__ __device__ __forceinline__ int row_at(int t) const
     {
         int j;
-        if (t <= c1) j = r0 + ceil_i(t, wf);
+        if (t <= c0 && c0 > 0) j = r0 + ceil_i(t, wz);
+        else if (t <= c1) j = b0 + ceil_i(t - c0, wf);
         else if (t <= c2) j = b1 + ceil_i(t - c1, wn);
         else j = b2 + ceil_i(t - c2, wd);
         return clampi(j, r0, r1);
@@ -374,6 +413,23 @@ __device__ __forceinline__ void strip_suffix_sums(const AggArgs& a, StripSmem<KT
     if (lane < R) sm.sk[lane][T] = 0.0;
 }
 
+// Half the spacing of the doubles at x, 2^(e - 53) for a normal positive x
+// of exponent e (x in [2^e, 2^(e+1))); 0 for anything else (zero, negative,
+// subnormal, too small for the half spacing to be a normal double, inf,
+// NaN).  Any 0 <= y < half_ulp(x) leaves x + y == x in round-to-nearest, and
+// so for every larger pivot too.
+__host__ __device__ __forceinline__ double half_ulp(double x)
+{
+    long long b;
+    __builtin_memcpy(&b, &x, sizeof b);
+    const int eb = (int)((b >> 52) & 0x7ff);
+    if (b <= 0 || eb >= 0x7ff || eb <= 53) return 0.0;
+    const long long hb = (long long)(eb - 53) << 52;
+    double h;
+    __builtin_memcpy(&h, &hb, sizeof h);
+    return h;
+}
+
 // One visit of column block q by the workgroup: chunks [c_lo, c_hi).  On
 // entry `cols` holds block q's column data and the LDS window is zero; on exit
 // `cols` holds block next_q's (if next_q >= 0, loaded before the window's
@@ -443,6 +499,7 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const StripPla
     double xk[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) xk[u] = sm.xw[lane + kWave * u];
+    const double hz = half_ulp(sm.xw[0]);
     // TK_r = sum over the block's columns of B_r(k) (the same in every lane)
     double TK[R];
     {
@@ -484,7 +541,7 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const StripPla
     // (a visit of far rows only splits them evenly: the same split, found
     // with less arithmetic)
     const VisitRows vrows(plan, q, vr0, vr1);
-    const bool far_only = vrows.b1 == vr1;
+    const bool far_only = vrows.b1 == vr1 && vrows.b0 == vr0;
     const int vcost = vrows.total();
     const int w_lo = far_only ? vr0 + (vr1 - vr0) * wv / Cols::NWV
                               : vrows.row_at(vcost * wv / Cols::NWV);
@@ -576,6 +633,24 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const StripPla
             }
             return __all(ok);
         };
+        // The piece's leading zero-birth rows: 0 <= x_j < half an ulp of the
+        // block's first pivot x_k0 (so of every pivot of the block), where
+        // x_j + x_k rounds to x_k and the far loop would add A_r(j) * 0 to
+        // every accumulator; their deaths are added above.  Counted on the
+        // rows in registers (x increases, so they are a prefix), rounded down
+        // to the loop's four-row step.
+        int fz = 0;
+        if (hz > 0.0) {
+            bool neg = false;
+            int cnt = 0;
+#pragma unroll
+            for (int t = 0; t < PR; ++t) {
+                const bool valid = rw0 + lane + kWave * t < rw1;
+                neg = neg || (valid && !(xj[t] >= 0.0));
+                cnt += __popcll(__ballot(valid && xj[t] >= 0.0 && xj[t] < hz));
+            }
+            fz = __any(neg) ? 0 : (cnt & ~3);
+        }
         const int nlow = min(nrows, max(0, k0 - rw0));
         int nf;
         if (nlow == 0) {
@@ -590,7 +665,8 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const StripPla
             }
             nf = lo;
         }
-        if (nf > 0) {
+        fz = min(fz, nf & ~3);
+        if (nf > fz) {
             // rows [0, nf): every pair lands in (k, k+1), or clips at column
             // n-1 (taken as weight w = 0 there: 1/(x_{k+1} - x_k) is 0 for
             // k = n-1).  Rows in pairs, the second row's update of an
@@ -624,8 +700,8 @@ __device__ __forceinline__ void agg_strip_visit(const AggArgs& a, const StripPla
                     for (int u = 0; u < U; ++u) acc[u][r] = fma(p1[r], w1[u], acc[u][r]);
             };
             double pa0[R + 1], pa1[R + 1], pb0[R + 1], pb1[R + 1];
-            ld2(0, pa0, pa1);
-            int jj = 0;
+            ld2(fz, pa0, pa1);
+            int jj = fz;
             for (; jj + 3 < nf; jj += 4) {
                 if ((jj & 31) == 28) pace(rw0 + jj);
                 ld2(jj + 2, pb0, pb1);

@@ -188,6 +188,27 @@ BREAKAGE_UPDATE_FLOPS = 8
 BREAKAGE_PARENT_FLOPS = 2
 
 
+def birth_pairs(x):
+    """Pairs j <= k of the grid x (increasing) whose birth weight is not
+    exactly zero: fl(x_j + x_k) != x_k.  A pair with fl(x_j + x_k) == x_k
+    (x_j below half an ulp of x_k) lands wholly in bin k, where its birth
+    cancels the column's loss, so it only takes its rate from bin j: a
+    separable death, O(n) work for all such pairs together.  For each k the
+    pairs form a suffix j >= j0(k) (the sum grows with x_j); j0 by a
+    vectorised bisection over all columns at once."""
+    x = np.asarray(x, dtype=np.float64)
+    n = len(x)
+    k = np.arange(n)
+    lo = np.zeros(n, dtype=np.int64)    # rows [0, lo) give x_j + x_k == x_k
+    hi = k.copy()                       # row k itself: 2 x_k != x_k
+    while np.any(lo < hi):
+        mid = (lo + hi) // 2
+        same = (x[mid] + x) == x
+        lo = np.where((lo < hi) & same, mid + 1, lo)
+        hi = np.where((lo < hi) & ~same, mid, hi)
+    return int(np.sum(k - lo + 1))
+
+
 def config_batched(count=4096, seed=4096, n_min=64, n_max=2048):
     """Batched workload: `count` independent instances, n log-uniform in
     [n_min, n_max] on geometric grids r = 1.03 from x_0 = 1e-18, each with its

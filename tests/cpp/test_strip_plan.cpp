@@ -1,6 +1,6 @@
 // test_strip_plan.cpp -- host check of the strip launch's work split
 // (agg_strip.hpp): for many sizes, block widths, band modes, near-chunk
-// counts and grid sizes, every chunk of every block goes to exactly one
+// counts, zero-birth chunk counts and grid sizes, every chunk of every block goes to exactly one
 // workgroup share, the cost line matches the sum of its chunk costs, and the
 // waves' row ranges of each visit tile the visit's rows in order.  Prints
 // one JSON line; exit status 0 when every check holds.  Runs without a GPU.
@@ -19,6 +19,7 @@ static int fails = 0;
 
 static long long chunk_cost(const StripPlan& s, int q, int c)
 {
+    if (c < strip_sz(s, q)) return s.cz;
     if (c < strip_sf(s, q)) return s.cf;
     if (c < strip_se(s, q)) return s.cn;
     return s.cd;
@@ -34,6 +35,7 @@ int main()
         for (int U : us)
             for (int band = 0; band < 2; ++band)
                 for (int m_near : {0, 3, 4, 9, 40})
+                    for (int zc : {-1, 0, 1, 5, 30, 100})
                     for (int G : gs) {
                         StripPlan s;
                         s.U = U;
@@ -41,10 +43,15 @@ int main()
                         s.band = band;
                         s.m_near = m_near;
                         s.mq = m_near / U + 1;
-                        s.cf = 4;
-                        s.cn = 10;
-                        s.cd = 6;
-                        s.cv = 20;
+                        // zc < 0: no zero-birth chunks; else m_zero = U + m_near + zc
+                        s.zb = zc >= 0;
+                        s.m_zero = zc >= 0 ? U + m_near + zc : 0;
+                        s.mz = s.m_zero / U + 1;
+                        s.cz = 1;
+                        s.cf = 20;
+                        s.cn = 50;
+                        s.cd = 30;
+                        s.cv = 100;
                         s.total = strip_total(s);
                         ++cases;
                         // cost line against the sum of its parts
@@ -57,6 +64,7 @@ int main()
                             for (int c = 0; c < nc; ++c) cs += chunk_cost(s, q, c);
                             tot += nc;
                             CHECK(strip_sf(s, q) <= strip_se(s, q) && strip_se(s, q) <= nc);
+                            CHECK(strip_sz(s, q) <= strip_sf(s, q));
                         }
                         CHECK(strip_cost_total(s) == cs);
                         CHECK(s.total == tot);

@@ -119,6 +119,26 @@ def test_daughter_tables():
         P.daughter_power_law(2.0, 7)
 
 
+def test_birth_pairs_brute_force():
+    """problems.birth_pairs (the pairs the roofline charges: fl(x_j + x_k) !=
+    x_k) against a pair-by-pair count on geometric grids wide enough that
+    most pairs have a zero birth weight, a jittered grid, and the edge of a
+    binade (x_j exactly half an ulp of x_k rounds to even)."""
+    import numpy as np
+    from pbe_hip import problems as P
+
+    def brute(x):
+        return sum(1 for k in range(len(x)) for j in range(k + 1) if x[j] + x[k] != x[k])
+
+    for x in (P.geometric_grid(500, 1e-18, 1.1), P.geometric_grid(300, 1e-18, 2 ** 0.25),
+              P.jittered_grid(400, 1e-18, 1.12, 0.2, seed=5)):
+        assert P.birth_pairs(x) == brute(x)
+    # 1 + 2^-53 rounds to 1 (tie to even), 1.5 ulp above 1 + 2^-52 does not
+    x = np.array([2.0 ** -54, 2.0 ** -53, 2.0 ** -52, 1.0, 1.0 + 2.0 ** -52])
+    assert P.birth_pairs(x) == brute(x)
+    assert P.birth_pairs(P.geometric_grid(16384, 1e-18, 1.02)) < 16384 * 16385 // 4
+
+
 def test_device_cube_root_host_check():
     """cbrt_pair (the bit-guess + Newton cube root of the kernels), compiled
     for the host from the same header: within 3 ulp of the C library's cbrt,

@@ -496,3 +496,42 @@ def test_band_every_split(pbe, oracle_mod, kernel):
             what = f"kernel {kernel} U={u} split {split} form {form}"
             assert_parity(got, ref, A, what=what)
             assert_m1(got, x, A, D, what=what)
+
+
+@pytest.mark.parametrize("kernel", [0, 3, 5])
+def test_zero_birth_rows(pbe, oracle_mod, kernel):
+    """Coarse geometric grids (r = 1.05, 1.1) where most strip rows lie below
+    half an ulp of their block's first pivot: x_j + x_k rounds to x_k, the
+    far-pair birth weight is exactly 0 and the strips only add those rows'
+    deaths.  Every strip launch shape, a jittered grid (declared not
+    geometric: the cost line prices no zero-birth chunks, the kernel still
+    skips them) and populations with random empty bins (bins fed only by
+    such pairs must stay exact)."""
+    from pbe_hip import hip, problems as P
+    rng = np.random.default_rng(70 + kernel)
+    for n, r, jitter, holes in ((6000, 1.05, 0.0, False), (3000, 1.1, 0.0, True),
+                                (6000, 1.05, 0.2, True)):
+        x = (P.geometric_grid(n, 1e-18, r) if jitter == 0.0
+             else P.jittered_grid(n, 1e-18, r, jitter, seed=n))
+        N = P.lognormal_ic(x, x[n // 3], 1.5, 1e12, noise=0.05, seed=80 + kernel)
+        # a broad floor keeps the far rows populated (the lognormal alone
+        # underflows there)
+        N = N + 1e-3 * rng.random(n)
+        if holes:
+            N[rng.random(n) < 0.3] = 0.0
+        log_x0, inv_log_r = P.grid_lookup(x) if jitter == 0.0 else (0.0, 0.0)
+        agg = dict(kernel=kernel, beta0=1e-17 if kernel != 2 else 1e-17 / x[n // 3],
+                   beta_br=6.73e-18, beta_sh=1e-3, log_x0=log_x0, inv_log_r=inv_log_r)
+        ref, A, D = _oracle(oracle_mod, N, x, agg=agg, threads=ORACLE_THREADS)
+        dN, dx = hip.DeviceArray.from_host(N), hip.DeviceArray.from_host(x)
+        drhs = hip.DeviceArray(n)
+        p = pbe.agg_params(n, agg)
+        for u, split, form in ((4, 4, 1), (4, 4, 2), (4, 8, 4), (2, 4, 1), (8, 4, 1), (0, 0, 0)):
+            drhs.zero()
+            hip.check(pbe.launch_aggregation_rhs_ex(dN, dx, drhs, p, tile_u=u, band_split=split,
+                                                    form=form))
+            hip.synchronize()
+            got = drhs.download()
+            what = f"zero-birth rows kernel {kernel} n={n} r={r} jitter {jitter} U={u} form {form}"
+            assert_parity(got, ref, A, what=what)
+            assert_m1(got, x, A, D, what=what)
